@@ -1,0 +1,151 @@
+"""Benchmark: plain GPU Voxel ReSTIR, area-light mode, on one MI355X.
+
+Workload (c2_area_1024): 1024^3 brick-grid world (value-noise heightfield x8
+plus 2,000 boxes), 4,096 emissive surface voxels with log-uniform intensity
+in [0.1, 10], per-face solid-angle light sampling, 1920x1080, 1 spp, M=32,
+temporal clamp 20x, 5 spatial taps within 30 px, along the seeded 60-frame
+spline camera path (frames wrap).  One step = one frame: primary brick-grid
+DDA -> 32 RIS candidates + shadow ray + temporal reuse -> spatial reuse ->
+shading with a second shadow ray.  The world, lights and all per-pixel state
+are resident in HBM before the timed region starts.
+
+Prints one JSON line (see the README of the benchmark contract in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "1080p fps & Mrays/s; equal-time MSE vs converged ref, learned vs plain"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per pixel and frame of each stage (SoA planes):
+#   primary: write hit word, dist, seed                               12
+#   initial: read hit 12, prev hit 8, prev reservoir 36;
+#            write reservoir 36, seed 4                                96
+#   spatial: read own reservoir 36 + hit 12; write reservoir 36, seed 4 88
+#   shade:   read hit word + dist 8, reservoir 36; write frame 16       60
+STAGE_BYTES = {"primary": 12, "initial": 96, "spatial": 88, "shade": 60}
+RAYS_PER_PIXEL = 3  # primary + candidate shadow ray + shading shadow ray
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, nthreads: int) -> dict:
+    """The oracle (CPU restatement of the reference kernels) on the host cores:
+    two consecutive frames of the same workload at full resolution, the
+    second (with temporal reuse) timed."""
+    import oracle as O
+    from smallproject_amd import configs
+    w = configs.build_world(sc)
+    view = O.View(w.grid(), w.bricks())
+    lut = np.array(list(w.cfg.emit_lut), np.float32)
+    ocfg = O.make_config(sc.width, sc.height, sc.point, gfxexp=1, quirks=sc.quirks, source=sc.source, lut=lut)
+    fr = O.Frames(ocfg, view, w.lights())
+    t = 0.0
+    for f in range(2):
+        pos, tgt = sc.camera(f)
+        w.look_at(pos, tgt)
+        p = w.prepare_frame()
+        w.finish_frame()
+        t0 = time.perf_counter()
+        fr.render(O.Params.from_buffer_copy(bytes(p)), threads=nthreads)
+        t = time.perf_counter() - t0
+        log(f"cpu_baseline: oracle frame {f} took {t:.2f} s on {nthreads} threads")
+    w.close()
+    return {"value": round(1.0 / t, 4), "unit": "frames/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle (CPU restatement of cl/ris.cl+trace.cl, OpenMP) rendering frames 0-1 of "
+                      f"{sc.name} at {sc.width}x{sc.height}; frame 1 (with temporal reuse) timed"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="c2_area_1024")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the render path runs on a single MI355X)", file=sys.stderr)
+        return 2
+
+    from smallproject_amd import configs
+    sc = configs.CONFIGS[args.config]
+    t0 = time.perf_counter()
+    w = configs.build_world(sc)
+    log(f"bench: built {sc.name} host world in {time.perf_counter() - t0:.1f} s "
+        f"({w.high_water()} bricks, {len(w.lights())} lights)")
+    w.force_sync()
+    r = w.renderer
+    frames_total = args.warmup + args.steps
+
+    def step(f: int) -> None:
+        pos, tgt = sc.camera(f % sc.frames)
+        w.look_at(pos, tgt)
+        w.render()
+
+    for f in range(args.warmup):
+        step(f)
+    r.sync()
+    log(f"bench: {args.warmup} warm-up frames done")
+    r.profile_begin()
+    r.sync()
+    t0 = time.perf_counter()
+    for f in range(args.warmup, frames_total):
+        step(f)
+    r.sync()
+    elapsed = time.perf_counter() - t0
+    stage_sum, nprof = r.profile_read()
+    log(f"bench: {args.steps} timed frames in {elapsed:.3f} s")
+
+    npx = sc.width * sc.height
+    fps = args.steps / elapsed
+    stage_ms = {name: float(stage_sum[i] / max(nprof, 1)) for i, name in enumerate(("primary", "initial", "spatial", "shade"))}
+    dom = max(stage_ms, key=stage_ms.get)
+    achieved = STAGE_BYTES[dom] * npx / (stage_ms[dom] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural world, lights and camera path)",
+        "config": {"workload": f"{sc.name}: plain Voxel ReSTIR, area lights, {sc.width}x{sc.height}, "
+                               f"1024^3 world, {len(w.lights())} emitters, M={sc.candidates}, "
+                               f"{sc.taps} taps r={sc.radius}, temporal clamp {sc.temporal_clamp}x",
+                   "resolution": [sc.width, sc.height], "frames_in_path": sc.frames},
+        "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_px": STAGE_BYTES[dom]},
+    }
+    w.close()
+    if not args.no_cpu_baseline:
+        n = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(sc, n)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,178 @@
+/*
+ * vr_api.h - C ABI of the MI355X Voxel ReSTIR render path.
+ *
+ * This library replaces what sits below World::Render()/World::Commit() in
+ * the reference: the OpenCL Kernel/Buffer runtime (template/template.cpp:
+ * 1266-1754) and the ReSTIR kernels of cl/ris.cl, cl/trace.cl, cl/tools.cl,
+ * cl/phat.cl, cl/reservoir.cl, cl/spatialSampling.cl and the commit/ubergrid
+ * kernels of cl/kernels.cl.  Above it, the C++ World class (world.h in the
+ * package) keeps the reference's host API and calls only these functions.
+ *
+ * Conventions: every function returns VR_OK (0) or a VR_ERR_* code and never
+ * throws; vr_last_error() describes the last failure of a context.  Pointers
+ * are host pointers unless a name says otherwise.  A context is not
+ * thread-safe: one host thread drives it (as the reference's single render
+ * thread does).  Rendering is asynchronous on the context's HIP stream;
+ * read-backs and vr_sync() block.
+ */
+#ifndef VR_API_H
+#define VR_API_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* world dimensions, template/common.h:16-21, 176-193 */
+#define VR_MAPDIM     1024u
+#define VR_BRICKDIM   8u
+#define VR_GRIDDIM    128u
+#define VR_UBERDIM    32u
+#define VR_GRIDSIZE   (128u * 128u * 128u)
+#define VR_BRICKSIZE  512u
+#define VR_BRICKCOUNT (1u << 20)
+#define VR_MAXCOMMITS 8192u
+
+/* status codes */
+#define VR_OK          0
+#define VR_ERR_ARG     1   /* bad argument (null pointer, size mismatch, range) */
+#define VR_ERR_HIP     2   /* a HIP runtime call failed */
+#define VR_ERR_STATE   3   /* call not valid in the current state */
+#define VR_ERR_NOMEM   4   /* device or host allocation failed */
+#define VR_ERR_UNSUPPORTED 5
+
+/* light modes: VOXELSAREPOINTLIGHTS (template/common.h:70) */
+#define VR_LIGHTS_AREA  0u
+#define VR_LIGHTS_POINT 1u
+
+/* candidate source pdf */
+#define VR_SOURCE_UNIFORM 0u  /* cl/tools.cl:593-598: p = 1/N (reference) */
+#define VR_SOURCE_POWER   1u  /* alias table over lightContribution (cl/tools.cl:579-591) */
+
+/* reference quirks reproduced when set (VR_QUIRKS_REFERENCE = all three) */
+#define VR_QUIRK_DOUBLE_RADIANCE 1u /* cl/ris.cl:254-255                      */
+#define VR_QUIRK_AXIS_ONLY_FACE  2u /* cl/ris.cl:93, cl/spatialSampling.cl:46 */
+#define VR_QUIRK_SHADOW_STEP_CAP 4u /* cl/ris.cl:10 (128-step shadow rays)    */
+#define VR_QUIRKS_REFERENCE      7u
+
+/* trace kinds for vr_trace_rays: TraceRay / TraceOpaqueRay (cl/trace.cl) */
+#define VR_TRACE_ALL    0u
+#define VR_TRACE_OPAQUE 1u
+
+/* stages, in pipeline order (used by vr_stage_times) */
+#define VR_STAGE_PRIMARY 0  /* renderAlbedo             cl/ris.cl:295-389 */
+#define VR_STAGE_INITIAL 1  /* perPixelInitialSampling  cl/ris.cl:20-124  */
+#define VR_STAGE_SPATIAL 2  /* perPixelSpatialResampling cl/ris.cl:126-151 */
+#define VR_STAGE_SHADE   3  /* renderRIS                cl/ris.cl:391-413 */
+#define VR_NUM_STAGES    4
+
+/* struct Light, template/common.h:134-139 */
+typedef struct vr_light {
+    uint32_t position; /* x + z*1024 + y*1024*1024 of the light's min corner */
+    uint32_t voxel;    /* voxel word 0x00MMARGB */
+    uint32_t size;     /* edge length in voxels (1 voxel light, 8 brick light) */
+} vr_light;
+
+/* struct RenderParams, template/common.h:100-118: the fields the ReSTIR
+ * path reads, same names and meaning.  float3 fields are packed (no
+ * OpenCL 16-byte padding).  Produced per frame by World::Render(). */
+typedef struct vr_render_params {
+    float oneOverRes[2];
+    float E[3], p0[3], p1[3], p2[3];
+    uint32_t R0, frame, framecount, restirtemporalframe;
+    float Nleft[4], Nright[4], Ntop[4], Nbottom[4];
+    int32_t accumulate, spatial, temporal, skyDomeSampling;
+    uint32_t numberOfLights, numberOfCandidates, numberOfMaxTemporalImportance;
+    uint32_t spatialRadius, spatialTaps;
+} vr_render_params;
+
+/* compile-time switches of the reference (template/common.h) as a
+ * per-context configuration */
+typedef struct vr_config {
+    uint32_t width, height;   /* SCRWIDTH, SCRHEIGHT */
+    uint32_t light_mode;      /* VR_LIGHTS_* */
+    uint32_t gfxexp;          /* GFXEXP: emission scaled by 1/pi, emitters also lit */
+    uint32_t quirks;          /* VR_QUIRK_* */
+    uint32_t source_pdf;      /* VR_SOURCE_* */
+    uint32_t brick_capacity;  /* device brick pool size in bricks (reference: VR_BRICKCOUNT) */
+    uint32_t flags;           /* reserved, 0 */
+    float emit_lut[256];      /* EmitStrength(v) = emit_lut[(v >> 16) & 255]; the
+                                 reference's is emit_lut[m] = m (common.h:208-211) */
+} vr_config;
+
+typedef struct vr_ctx vr_ctx;
+
+/* Fill *cfg with the reference defaults for a width x height view:
+ * area lights, GFXEXP=1, all quirks, uniform source, 2^20 bricks,
+ * identity emission LUT. */
+void vr_default_config(vr_config* cfg, uint32_t width, uint32_t height);
+
+/* Create a render context on the current HIP device: allocates the top grid
+ * (8 MiB), uber grid, brick pool (brick_capacity * 2 KiB), two hit buffers
+ * (12 B/px each), two reservoir buffers (36 B/px each) and the frame (16 B/px).
+ * Replaces World::World's buffer set-up (template/world.cpp:47-193). */
+int vr_create(const vr_config* cfg, vr_ctx** out);
+int vr_destroy(vr_ctx* ctx);
+const char* vr_last_error(const vr_ctx* ctx);
+
+/* World::Commit device side (template/world.cpp:2357-2420, cl/kernels.cl:
+ * 448-513): copy the full top grid (VR_GRIDSIZE u32, index bx+bz*128+by*16384),
+ * scatter n changed bricks (ids[n], payload[n*512]) into the pool, rebuild
+ * the uber grid.  n may exceed VR_MAXCOMMITS (uploaded in several batches). */
+int vr_commit(vr_ctx* ctx, const uint32_t* grid, const uint32_t* brick_ids,
+              const uint32_t* brick_payload, uint32_t n);
+
+/* Upload the light list (World::SetupLightBuffer, template/world.cpp:563-612). */
+int vr_set_lights(vr_ctx* ctx, const vr_light* lights, uint32_t n);
+
+/* World::Render device side (template/world.cpp:2127-2340): swap the hit
+ * buffers, then primary hit -> candidates + visibility + temporal reuse ->
+ * spatial reuse -> shading.  Asynchronous. */
+int vr_render_frame(vr_ctx* ctx, const vr_render_params* params);
+
+/* Block until all work of the context has finished. */
+int vr_sync(vr_ctx* ctx);
+
+/* Read-backs (blocking), n = width*height.
+ * hits (current frame): hitword = voxel | side<<24 | (D[side] > 0)<<26,
+ * dist, seed.  Any output pointer may be NULL. */
+int vr_read_hits(vr_ctx* ctx, uint32_t* hitword, float* dist, uint32_t* seed);
+/* which = 0: the final reservoirs of the last frame (after spatial reuse and
+ * shading's discard), 1: the output of candidate sampling + temporal reuse.
+ * a = (wsum, W, pHat, invPdf) per pixel, b = (pos.xyz, lightIndex bits),
+ * m = M | Nlight face << 24 (face 0..5 = +x,+y,+z,-x,-y,-z; 7 = none). */
+int vr_read_reservoirs(vr_ctx* ctx, int which, float* a4, float* b4, uint32_t* m);
+/* frame: rgb radiance and hit distance per pixel (cl/ris.cl:292, 411) */
+int vr_read_frame(vr_ctx* ctx, float* rgba);
+
+/* Per-stage device time of the last rendered frame, milliseconds, measured
+ * with HIP events on the render stream (template/world.cpp:2439-2471). */
+int vr_stage_times(vr_ctx* ctx, float ms[VR_NUM_STAGES]);
+
+/* Stage profiling over many frames (the reference keeps 100-frame averages,
+ * template/world.cpp:2342-2353): vr_profile_begin starts summing every
+ * frame's per-stage event times; vr_profile_read waits for the profiled
+ * frames and returns the sums (ms) and the number of frames. */
+int vr_profile_begin(vr_ctx* ctx);
+int vr_profile_read(vr_ctx* ctx, double sum_ms[VR_NUM_STAGES], uint32_t* frames);
+
+/* Trace n rays through the committed world (World::TraceBatch,
+ * template/world.cpp:2051-2068, kernel traceBatch cl/kernels.cl:395-417).
+ * origins/dirs: n*3 floats; kind VR_TRACE_*; steps: DDA step budget.
+ * Outputs voxel, dist, side per ray (voxel 0 = miss, dist/side then 0). */
+int vr_trace_rays(vr_ctx* ctx, const float* origins, const float* dirs, uint32_t n,
+                  uint32_t kind, int32_t steps, uint32_t* voxel, float* dist, uint32_t* side);
+
+/* FP64 converged reference image: render_di_gfxexp (cl/kernels.cl:177-241)
+ * accumulated over spp samples in binary64 (the reference's ACCUMULATOR=1
+ * path, cl/kernels.cl:304-326, keeps an FP32 running mean instead).
+ * Sample f uses R0 = xorshift32^(f+1)(params->R0) and strata frame
+ * (frame0 + f) & 255.  accum: width*height*3 doubles, added to (host). */
+int vr_ground_truth(vr_ctx* ctx, const vr_render_params* params, uint32_t spp,
+                    uint32_t frame0, double* accum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,107 @@
+/*
+ * vr_world.h - C ABI over the host World (the reference's WrldTmpl8 World
+ * class, template/world.h:164-758, and worldapi, template/worldapi.h:1-101),
+ * plus the seeded scene generators and camera paths of the benchmark
+ * configurations.
+ *
+ * A vrw_world owns the authoritative host copy of the brick grid, brick pool
+ * and light list and a lazily created vr_ctx (include/vr_api.h) that mirrors
+ * them on the device.  Editing calls (vrw_set, vrw_add_brick, ...) touch host
+ * memory only and work without a GPU; vrw_commit/vrw_force_sync/vrw_render
+ * create the device context on first use.
+ */
+#ifndef VR_WORLD_H
+#define VR_WORLD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#include "vr_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vrw_world vrw_world;
+
+/* World::World (template/world.cpp:47-193) + InitReSTIR (195-208) */
+int vrw_create(const vr_config* cfg, vrw_world** out);
+int vrw_destroy(vrw_world* w);
+const char* vrw_last_error(const vrw_world* w);
+/* the device context (created on first call); NULL if no GPU is usable */
+vr_ctx* vrw_context(vrw_world* w);
+
+/* ClearWorld / Plot / Read (template/worldapi.h:5-16) and World::Set/Get/
+ * AddBrick/RemoveBrick/OptimizeBricks (template/world.h:293-513,
+ * template/world.cpp:655-683) */
+void vrw_clear(vrw_world* w);
+uint32_t vrw_get(const vrw_world* w, uint32_t x, uint32_t y, uint32_t z);
+void vrw_set(vrw_world* w, uint32_t x, uint32_t y, uint32_t z, uint32_t v);
+/* n voxels at xyz[3n] with values v[n], applied in order */
+void vrw_set_many(vrw_world* w, const uint32_t* xyz, const uint32_t* v, size_t n);
+void vrw_add_brick(vrw_world* w, uint32_t bx, uint32_t by, uint32_t bz, uint32_t v);
+void vrw_remove_brick(vrw_world* w, uint32_t bx, uint32_t by, uint32_t bz);
+void vrw_optimize(vrw_world* w);
+/* Box (template/template.cpp:121-124): x1..x2-1 etc. */
+void vrw_box(vrw_world* w, int x1, int y1, int z1, int x2, int y2, int z2, uint32_t c);
+void vrw_sphere(vrw_world* w, float x, float y, float z, float r, uint32_t c);
+
+/* lights (template/world.cpp:228-612) */
+uint32_t vrw_find_lights(const vrw_world* w, vr_light* out, uint32_t cap);
+uint32_t vrw_num_lights(const vrw_world* w);
+uint32_t vrw_get_lights(const vrw_world* w, vr_light* out, uint32_t cap);
+void vrw_setup_light_buffer(vrw_world* w, const vr_light* ls, uint32_t n, int pos);
+void vrw_add_light(vrw_world* w, int x, int y, int z, uint32_t size, uint32_t c);
+void vrw_remove_light(vrw_world* w, int x, int y, int z);
+
+/* host data (template/world.h:207-211) */
+const uint32_t* vrw_grid(const vrw_world* w);
+const uint32_t* vrw_bricks(const vrw_world* w);
+const uint32_t* vrw_zeroes(const vrw_world* w);
+uint32_t vrw_brick_high_water(const vrw_world* w);
+uint32_t vrw_dirty_bricks(const vrw_world* w, uint32_t* ids, uint32_t cap);
+
+/* camera: LookAt (template/template.cpp:204-215) and the focal distance of
+ * the pinhole (2.2 in template/world.cpp:2155-2157; 1/tan(fov/2) in general) */
+void vrw_look_at(vrw_world* w, const float pos[3], const float target[3]);
+void vrw_set_focal(vrw_world* w, float focal);
+/* the World's RenderParams (mutable, like World::GetRenderParams()) */
+vr_render_params* vrw_params(vrw_world* w);
+/* R0 of the next frame (otherwise the World's xorshift32, seed 0x12345678) */
+void vrw_set_next_r0(vrw_world* w, uint32_t r0);
+
+/* per frame (template/world.cpp:2127-2483) */
+int vrw_render(vrw_world* w);
+/* Render() without the launch: vrw_prepare_frame fills vrw_params() for the
+ * next frame (camera, reprojection planes, R0, light count), vrw_finish_frame
+ * advances the frame counters */
+void vrw_prepare_frame(vrw_world* w);
+void vrw_finish_frame(vrw_world* w);
+int vrw_commit(vrw_world* w);
+int vrw_force_sync(vrw_world* w);
+int vrw_stage_times(vrw_world* w, float ms[VR_NUM_STAGES]);
+
+/* ---- seeded scene generators of the benchmark configurations ---- */
+/* value-noise heightfield over [0,size)^2: h = scale*(40 + 24*n(x/scale, z/scale))
+ * with n a 4-octave value noise normalised to [-1,1] */
+float vrw_terrain_height(float x, float z, uint32_t scale, uint32_t seed);
+int vrw_scene_terrain(vrw_world* w, uint32_t size, uint32_t scale, uint32_t seed);
+/* n axis-aligned boxes, edges uniform in [emin, emax], resting in the terrain */
+int vrw_scene_boxes(vrw_world* w, uint32_t n, uint32_t emin, uint32_t emax, uint32_t size, uint32_t scale,
+                    uint32_t terrain_seed, uint32_t seed);
+/* n emissive voxels at uniform (x,z), hmin..hmax voxels above the terrain,
+ * emit byte uniform in [mmin, mmax] */
+int vrw_scene_lights_above(vrw_world* w, uint32_t n, uint32_t size, uint32_t scale, uint32_t terrain_seed,
+                           uint32_t seed, uint32_t hmin, uint32_t hmax, uint32_t mmin, uint32_t mmax);
+/* n emissive voxels replacing the top voxel of uniformly drawn columns */
+int vrw_scene_lights_surface(vrw_world* w, uint32_t n, uint32_t size, uint32_t seed, uint32_t mmin, uint32_t mmax);
+/* emission LUT: lut[0] = 0, lut[m] = lo * (hi/lo)^((m-1)/254) for m >= 1 */
+void vrw_log_lut(float lo, float hi, float lut[256]);
+/* camera path: kind 0 = straight line at 2 voxels/frame toward the centre,
+ * kind 1 = Catmull-Rom spline through seeded control points above the terrain */
+int vrw_camera_path(uint32_t kind, uint32_t seed, uint32_t frame, uint32_t nframes, uint32_t size, uint32_t scale,
+                    uint32_t terrain_seed, float pos[3], float target[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

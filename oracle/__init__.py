@@ -1,0 +1,328 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  The oracle restates the reference's Voxel ReSTIR
+path on the CPU (oracle/vr_oracle.c) and is the checker for the product's
+HIP kernels.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBPATH = HERE / "liboracle.so"
+
+TRACE_ALL, TRACE_OPAQUE, TRACE_THROUGH = 0, 1, 2
+QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
+QUIRKS_REFERENCE = 7
+SOURCE_UNIFORM, SOURCE_POWER = 0, 1
+
+RAY_DTYPE = np.dtype([("voxel", "<u4"), ("dist", "<f4"), ("side", "<u4"), ("seed", "<u4"), ("D", "<f4", (3,))])
+RES_DTYPE = np.dtype([("wsum", "<f4"), ("M", "<u4"), ("lightIndex", "<u4"), ("W", "<f4"), ("pos", "<f4", (3,)),
+                      ("invPdf", "<f4"), ("Nlight", "<f4", (3,)), ("pHat", "<f4")])
+LIGHT_DTYPE = np.dtype([("position", "<u4"), ("voxel", "<u4"), ("size", "<u4")])
+
+
+class Params(C.Structure):
+    """struct RenderParams subset (template/common.h:100-118)."""
+    _fields_ = [("oneOverRes", C.c_float * 2), ("E", C.c_float * 3), ("p0", C.c_float * 3),
+                ("p1", C.c_float * 3), ("p2", C.c_float * 3), ("R0", C.c_uint32), ("frame", C.c_uint32),
+                ("framecount", C.c_uint32), ("restirtemporalframe", C.c_uint32), ("Nleft", C.c_float * 4),
+                ("Nright", C.c_float * 4), ("Ntop", C.c_float * 4), ("Nbottom", C.c_float * 4),
+                ("accumulate", C.c_int32), ("spatial", C.c_int32), ("temporal", C.c_int32),
+                ("skyDomeSampling", C.c_int32), ("numberOfLights", C.c_uint32), ("numberOfCandidates", C.c_uint32),
+                ("numberOfMaxTemporalImportance", C.c_uint32), ("spatialRadius", C.c_uint32),
+                ("spatialTaps", C.c_uint32)]
+
+
+class Config(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("point_lights", C.c_uint32),
+                ("gfxexp", C.c_uint32), ("quirks", C.c_uint32), ("source_pdf", C.c_uint32),
+                ("emit_lut", C.c_float * 256)]
+
+
+class WorldView(C.Structure):
+    _fields_ = [("grid", C.c_void_p), ("bricks", C.c_void_p), ("uber", C.c_void_p)]
+
+
+class Alias(C.Structure):
+    _fields_ = [("prob", C.c_void_p), ("alias", C.c_void_p), ("pmf", C.c_void_p)]
+
+
+class State(C.Structure):
+    _fields_ = [("W", C.c_uint32), ("H", C.c_uint32), ("hits", C.c_void_p * 2), ("res", C.c_void_p * 2),
+                ("frame", C.c_void_p), ("hitCur", C.c_int), ("hitPrev", C.c_int), ("resOut", C.c_int),
+                ("resIn", C.c_int), ("initialOut", C.c_int), ("spatialOut", C.c_int), ("shadeIn", C.c_int)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load liboracle.so (built by oracle/build.py)."""
+    global _lib
+    if _lib is None:
+        if not LIBPATH.exists():
+            raise RuntimeError(f"{LIBPATH} missing: run oracle/build.py")
+        L = C.CDLL(str(LIBPATH))
+        P = C.c_void_p
+        u32, f32, i32 = C.c_uint32, C.c_float, C.c_int
+        sig = {
+            "or_wang_hash": (u32, [u32]),
+            "or_random_int": (u32, [P]),
+            "or_random_float": (f32, [P]),
+            "or_get_alphaf": (f32, [u32]),
+            "or_to_float_rgb": (None, [u32, P]),
+            "or_from_float_rgba": (u32, [P, u32]),
+            "or_sincos": (None, [f32, P, P]),
+            "or_camera_ray": (None, [P, i32, i32, P]),
+            "or_trace": (u32, [P, P, P, i32, i32, u32, u32, P, P]),
+            "or_trace_batch": (None, [P, P, P, u32, i32, i32, P, P, P]),
+            "or_random_point_on_voxel": (None, [P, P, i32, P, P, P, P]),
+            "or_eval_phat": (f32, [P, P, P, P, P, P, u32, u32]),
+            "or_eval_phat_full": (None, [P, P, P, P, P, P, u32, u32, P]),
+            "or_reservoir_update": (None, [P, f32, f32, u32, f32, P, f32, P]),
+            "or_reservoir_combine": (None, [P, P, f32]),
+            "or_reservoir_reweigh": (None, [P, f32, u32]),
+            "or_reservoir_adjust": (None, [P, f32]),
+            "or_reproject": (None, [P, u32, u32, P, P]),
+            "or_build_power_table": (i32, [P, P, u32, P, P, P]),
+            "or_stage_albedo": (None, [P, P, P, P, i32, i32]),
+            "or_state_create": (P, [u32, u32]),
+            "or_state_destroy": (None, [P]),
+            "or_render_frame": (None, [P, P, P, P, P, P, i32]),
+            "or_ground_truth": (None, [P, P, P, u32, u32, P, i32, i32]),
+            "or_world_create": (P, [u32]),
+            "or_world_destroy": (None, [P]),
+            "or_world_clear": (None, [P]),
+            "or_world_get": (u32, [P, u32, u32, u32]),
+            "or_world_set": (None, [P, u32, u32, u32, u32]),
+            "or_world_add_brick": (None, [P, u32, u32, u32, u32]),
+            "or_world_remove_brick": (None, [P, u32, u32, u32]),
+            "or_world_optimize": (u32, [P]),
+            "or_world_find_lights": (u32, [P, P, u32]),
+            "or_world_num_lights": (u32, [P]),
+            "or_world_lights": (P, [P]),
+            "or_world_grid": (P, [P]),
+            "or_world_bricks": (P, [P]),
+            "or_world_zeroes": (P, [P]),
+            "or_world_brick_high_water": (u32, [P]),
+            "or_world_dirty": (u32, [P, P, u32]),
+            "or_build_uber": (None, [P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def make_config(width: int, height: int, point: bool, gfxexp: int = 1, quirks: int = QUIRKS_REFERENCE,
+                source: int = SOURCE_UNIFORM, lut=None) -> Config:
+    c = Config()
+    c.width, c.height, c.point_lights, c.gfxexp, c.quirks, c.source_pdf = width, height, int(point), gfxexp, quirks, source
+    lut = np.arange(256, dtype=np.float32) if lut is None else np.asarray(lut, dtype=np.float32)
+    for i in range(256):
+        c.emit_lut[i] = float(lut[i])
+    return c
+
+
+# ---- scalar helpers -------------------------------------------------------
+def wang_hash(s: int) -> int:
+    return lib().or_wang_hash(s & 0xFFFFFFFF)
+
+
+def xorshift(s: int) -> tuple[int, int]:
+    st = np.array([s & 0xFFFFFFFF], dtype=np.uint32)
+    v = lib().or_random_int(ptr(st))
+    return v, int(st[0])
+
+
+def random_float(s: int) -> tuple[float, int]:
+    st = np.array([s & 0xFFFFFFFF], dtype=np.uint32)
+    v = lib().or_random_float(ptr(st))
+    return v, int(st[0])
+
+
+def sincos(angle: float) -> tuple[float, float]:
+    s = np.zeros(1, np.float32)
+    c = np.zeros(1, np.float32)
+    lib().or_sincos(C.c_float(angle), ptr(s), ptr(c))
+    return float(s[0]), float(c[0])
+
+
+# ---- worlds ---------------------------------------------------------------
+class OracleWorld:
+    """The reference World's build semantics (template/world.h:293-606)."""
+
+    def __init__(self, capacity: int = 1 << 20):
+        self.cap = capacity
+        self.h = lib().or_world_create(capacity)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_world_destroy(self.h)
+            self.h = None
+
+    def set(self, x, y, z, v):
+        lib().or_world_set(self.h, x, y, z, v)
+
+    def get(self, x, y, z) -> int:
+        return lib().or_world_get(self.h, x, y, z)
+
+    def add_brick(self, bx, by, bz, v):
+        lib().or_world_add_brick(self.h, bx, by, bz, v)
+
+    def remove_brick(self, bx, by, bz):
+        lib().or_world_remove_brick(self.h, bx, by, bz)
+
+    def optimize(self) -> int:
+        return lib().or_world_optimize(self.h)
+
+    def clear(self):
+        lib().or_world_clear(self.h)
+
+    def grid(self) -> np.ndarray:
+        return np.ctypeslib.as_array(C.cast(lib().or_world_grid(self.h), C.POINTER(C.c_uint32)), (128 ** 3,)).copy()
+
+    def high_water(self) -> int:
+        return lib().or_world_brick_high_water(self.h)
+
+    def bricks(self) -> np.ndarray:
+        n = self.high_water() * 512
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        return np.ctypeslib.as_array(C.cast(lib().or_world_bricks(self.h), C.POINTER(C.c_uint32)), (n,)).copy()
+
+    def zeroes(self) -> np.ndarray:
+        return np.ctypeslib.as_array(C.cast(lib().or_world_zeroes(self.h), C.POINTER(C.c_uint32)), (self.cap,)).copy()
+
+    def lights(self) -> np.ndarray:
+        n = lib().or_world_num_lights(self.h)
+        if n == 0:
+            return np.zeros(0, LIGHT_DTYPE)
+        return np.ctypeslib.as_array(C.cast(lib().or_world_lights(self.h), C.POINTER(C.c_uint32)), (n * 3,)).copy().view(LIGHT_DTYPE)
+
+    def find_lights(self) -> np.ndarray:
+        n = lib().or_world_find_lights(self.h, None, 0)
+        out = np.zeros(n, LIGHT_DTYPE)
+        if n:
+            lib().or_world_find_lights(self.h, ptr(out), n)
+        return out
+
+    def dirty(self) -> np.ndarray:
+        n = lib().or_world_dirty(self.h, None, 0)
+        out = np.zeros(n, np.uint32)
+        if n:
+            lib().or_world_dirty(self.h, ptr(out), n)
+        return out
+
+
+def build_uber(grid: np.ndarray) -> np.ndarray:
+    uber = np.zeros(32 ** 3, np.uint8)
+    g = np.ascontiguousarray(grid, dtype=np.uint32)
+    lib().or_build_uber(ptr(g), ptr(uber))
+    return uber
+
+
+class View:
+    """Keeps the arrays of an or_world_view alive."""
+
+    def __init__(self, grid: np.ndarray, bricks: np.ndarray, uber: np.ndarray | None = None):
+        self.grid = np.ascontiguousarray(grid, dtype=np.uint32)
+        self.bricks = np.ascontiguousarray(bricks, dtype=np.uint32)
+        if self.bricks.size == 0:
+            self.bricks = np.zeros(512, np.uint32)
+        self.uber = build_uber(self.grid) if uber is None else np.ascontiguousarray(uber, dtype=np.uint8)
+        self.s = WorldView(ptr(self.grid), ptr(self.bricks), ptr(self.uber))
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def trace_batch(view: View, O: np.ndarray, D: np.ndarray, steps: int = 999999, kind: int = TRACE_ALL):
+    O = np.ascontiguousarray(O, dtype=np.float32).reshape(-1, 3)
+    D = np.ascontiguousarray(D, dtype=np.float32).reshape(-1, 3)
+    n = O.shape[0]
+    v = np.zeros(n, np.uint32)
+    d = np.zeros(n, np.float32)
+    s = np.zeros(n, np.uint32)
+    lib().or_trace_batch(view.ref, ptr(O), ptr(D), n, steps, kind, ptr(v), ptr(d), ptr(s))
+    return v, d, s
+
+
+def power_table(cfg: Config, lights: np.ndarray):
+    n = len(lights)
+    prob = np.zeros(max(n, 1), np.float32)
+    alias = np.zeros(max(n, 1), np.uint32)
+    pmf = np.zeros(max(n, 1), np.float32)
+    L = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
+    lib().or_build_power_table(C.byref(cfg), ptr(L) if n else None, n, ptr(prob), ptr(alias), ptr(pmf))
+    return prob, alias, pmf
+
+
+class Frames:
+    """Oracle frame state: World::Render's buffer rotation (world.cpp:2209-2223)."""
+
+    def __init__(self, cfg: Config, view: View, lights: np.ndarray):
+        self.cfg = cfg
+        self.view = view
+        self.W, self.H = cfg.width, cfg.height
+        self.h = lib().or_state_create(self.W, self.H)
+        self.st = C.cast(self.h, C.POINTER(State)).contents
+        self.lights = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
+        if len(self.lights) == 0:
+            self.lights = np.zeros(1, LIGHT_DTYPE)
+        self.alias = None
+        if cfg.source_pdf == SOURCE_POWER and len(lights):
+            self.tables = power_table(cfg, lights)
+            self.alias = Alias(ptr(self.tables[0]), ptr(self.tables[1]), ptr(self.tables[2]))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_state_destroy(self.h)
+            self.h = None
+
+    def render(self, params: Params, threads: int = 8):
+        lib().or_render_frame(self.h, C.byref(self.cfg), C.byref(params), self.view.ref, ptr(self.lights),
+                              C.byref(self.alias) if self.alias is not None else None, threads)
+
+    def _arr(self, p, dtype):
+        n = self.W * self.H
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), (n * dtype.itemsize,)).view(dtype).copy()
+
+    def hits(self) -> np.ndarray:
+        return self._arr(self.st.hits[self.st.hitCur], RAY_DTYPE)
+
+    def prev_hits(self) -> np.ndarray:
+        return self._arr(self.st.hits[self.st.hitPrev], RAY_DTYPE)
+
+    def initial(self) -> np.ndarray:
+        """Reservoirs after candidates + temporal reuse of the last frame."""
+        return self._arr(self.st.res[self.st.initialOut], RES_DTYPE)
+
+    def final(self) -> np.ndarray:
+        """Reservoirs after spatial reuse and shading's discard."""
+        return self._arr(self.st.res[self.st.shadeIn], RES_DTYPE)
+
+    def frame(self) -> np.ndarray:
+        n = self.W * self.H
+        return np.ctypeslib.as_array(C.cast(self.st.frame, C.POINTER(C.c_float)), (n * 4,)).copy().reshape(-1, 4)
+
+
+def ground_truth(cfg: Config, params: Params, view: View, spp: int, frame0: int = 0,
+                 y0: int = 0, y1: int | None = None) -> np.ndarray:
+    acc = np.zeros(cfg.width * cfg.height * 3, np.float64)
+    y1 = cfg.height if y1 is None else y1
+    lib().or_ground_truth(C.byref(cfg), C.byref(params), view.ref, spp, frame0, ptr(acc), y0, y1)
+    return acc.reshape(-1, 3)

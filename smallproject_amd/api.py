@@ -1,0 +1,308 @@
+"""Python mirror of the World host API and of the render context.
+
+`World` drives the C++ World (include/vr_world.h) the way the reference's
+game code drives template/world.h: build with set/add_brick, optimize,
+look_at, then render()/commit() per frame.  `Renderer` is the bare render
+context (include/vr_api.h) for callers that hold their own world arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import (LIGHT_DTYPE, VrConfig, VrLight, VrRenderParams, check, lib, ptr)
+
+
+def default_config(width: int, height: int, *, point: bool = False, quirks: int = _lib.QUIRKS_REFERENCE,
+                   gfxexp: int = 1, source: int = _lib.SOURCE_UNIFORM, capacity: int = _lib.BRICKCOUNT,
+                   lut=None) -> VrConfig:
+    cfg = VrConfig()
+    lib().vr_default_config(C.byref(cfg), width, height)
+    cfg.light_mode = _lib.LIGHTS_POINT if point else _lib.LIGHTS_AREA
+    cfg.quirks, cfg.gfxexp, cfg.source_pdf, cfg.brick_capacity = quirks, gfxexp, source, capacity
+    if lut is not None:
+        lut = np.asarray(lut, dtype=np.float32)
+        for i in range(256):
+            cfg.emit_lut[i] = float(lut[i])
+    return cfg
+
+
+def log_lut(lo: float, hi: float) -> np.ndarray:
+    out = np.zeros(256, np.float32)
+    lib().vrw_log_lut(C.c_float(lo), C.c_float(hi), ptr(out))
+    return out
+
+
+def focal_for_fov(fov_deg: float) -> float:
+    """Pinhole focal distance for a vertical field of view (the reference's 2.2 is 48.9 deg)."""
+    return 1.0 / math.tan(math.radians(fov_deg) * 0.5)
+
+
+class Renderer:
+    """A vr_ctx: device world + per-pixel state + frame pipeline."""
+
+    def __init__(self, cfg: VrConfig):
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().vr_create(C.byref(cfg), C.byref(h)), None, "vr_create")
+        self.h = h.value
+        self.n = cfg.width * cfg.height
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vr_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, code, what):
+        check(code, self.h, what)
+
+    def commit(self, grid: np.ndarray, ids: np.ndarray | None = None, payload: np.ndarray | None = None):
+        grid = np.ascontiguousarray(grid, dtype=np.uint32)
+        assert grid.size == _lib.GRIDSIZE
+        n = 0 if ids is None else int(len(ids))
+        ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        payload = None if payload is None else np.ascontiguousarray(payload, dtype=np.uint32)
+        if n:
+            assert payload.size == n * _lib.BRICKSIZE
+        self._chk(lib().vr_commit(self.h, ptr(grid), ptr(ids), ptr(payload), n), "vr_commit")
+
+    def set_lights(self, lights: np.ndarray):
+        L = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
+        self._chk(lib().vr_set_lights(self.h, ptr(L) if len(L) else None, len(L)), "vr_set_lights")
+
+    def render(self, params: VrRenderParams):
+        self._chk(lib().vr_render_frame(self.h, C.byref(params)), "vr_render_frame")
+
+    def sync(self):
+        self._chk(lib().vr_sync(self.h), "vr_sync")
+
+    def hits(self):
+        hw = np.zeros(self.n, np.uint32)
+        d = np.zeros(self.n, np.float32)
+        s = np.zeros(self.n, np.uint32)
+        self._chk(lib().vr_read_hits(self.h, ptr(hw), ptr(d), ptr(s)), "vr_read_hits")
+        return hw, d, s
+
+    def reservoirs(self, which: int):
+        a = np.zeros((self.n, 4), np.float32)
+        b = np.zeros((self.n, 4), np.float32)
+        m = np.zeros(self.n, np.uint32)
+        self._chk(lib().vr_read_reservoirs(self.h, which, ptr(a), ptr(b), ptr(m)), "vr_read_reservoirs")
+        return a, b, m
+
+    def frame(self) -> np.ndarray:
+        f = np.zeros((self.n, 4), np.float32)
+        self._chk(lib().vr_read_frame(self.h, ptr(f)), "vr_read_frame")
+        return f
+
+    def stage_times(self) -> np.ndarray:
+        ms = np.zeros(_lib.NUM_STAGES, np.float32)
+        self._chk(lib().vr_stage_times(self.h, ptr(ms)), "vr_stage_times")
+        return ms
+
+    def profile_begin(self):
+        self._chk(lib().vr_profile_begin(self.h), "vr_profile_begin")
+
+    def profile_read(self):
+        """(per-stage summed ms, frames) since profile_begin."""
+        s = np.zeros(_lib.NUM_STAGES, np.float64)
+        n = C.c_uint32()
+        self._chk(lib().vr_profile_read(self.h, ptr(s), C.byref(n)), "vr_profile_read")
+        return s, n.value
+
+    def trace(self, O: np.ndarray, D: np.ndarray, kind: int = _lib.TRACE_ALL, steps: int = 999999):
+        O = np.ascontiguousarray(O, dtype=np.float32).reshape(-1, 3)
+        D = np.ascontiguousarray(D, dtype=np.float32).reshape(-1, 3)
+        n = O.shape[0]
+        v = np.zeros(n, np.uint32)
+        d = np.zeros(n, np.float32)
+        s = np.zeros(n, np.uint32)
+        self._chk(lib().vr_trace_rays(self.h, ptr(O), ptr(D), n, kind, steps, ptr(v), ptr(d), ptr(s)), "vr_trace_rays")
+        return v, d, s
+
+    def ground_truth(self, params: VrRenderParams, spp: int, frame0: int = 0, accum: np.ndarray | None = None):
+        acc = np.zeros((self.n, 3), np.float64) if accum is None else accum
+        self._chk(lib().vr_ground_truth(self.h, C.byref(params), spp, frame0, ptr(acc)), "vr_ground_truth")
+        return acc
+
+
+class World:
+    """The reference's World (template/world.h) over the C++ host library."""
+
+    def __init__(self, cfg: VrConfig):
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().vrw_create(C.byref(cfg), C.byref(h)), None, "vrw_create")
+        self.h = h.value
+        self._renderer = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vrw_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, code: int, what: str):
+        if code != _lib.VR_OK:
+            m = lib().vrw_last_error(self.h)
+            raise _lib.VrError(code, f"{what}: {m.decode() if m else ''}")
+
+    # voxel access
+    def set(self, x, y, z, v):
+        lib().vrw_set(self.h, x, y, z, v)
+
+    def set_many(self, xyz: np.ndarray, v: np.ndarray):
+        xyz = np.ascontiguousarray(xyz, dtype=np.uint32).reshape(-1, 3)
+        v = np.ascontiguousarray(v, dtype=np.uint32).reshape(-1)
+        assert xyz.shape[0] == v.shape[0]
+        lib().vrw_set_many(self.h, ptr(xyz), ptr(v), v.shape[0])
+
+    def get(self, x, y, z) -> int:
+        return lib().vrw_get(self.h, x, y, z)
+
+    def add_brick(self, bx, by, bz, v):
+        lib().vrw_add_brick(self.h, bx, by, bz, v)
+
+    def remove_brick(self, bx, by, bz):
+        lib().vrw_remove_brick(self.h, bx, by, bz)
+
+    def optimize(self):
+        lib().vrw_optimize(self.h)
+
+    def clear(self):
+        lib().vrw_clear(self.h)
+
+    def box(self, x1, y1, z1, x2, y2, z2, c):
+        lib().vrw_box(self.h, x1, y1, z1, x2, y2, z2, c)
+
+    # lights
+    def lights(self) -> np.ndarray:
+        n = lib().vrw_num_lights(self.h)
+        out = np.zeros(n, LIGHT_DTYPE)
+        if n:
+            lib().vrw_get_lights(self.h, ptr(out), n)
+        return out
+
+    def find_lights(self) -> np.ndarray:
+        n = lib().vrw_find_lights(self.h, None, 0)
+        out = np.zeros(n, LIGHT_DTYPE)
+        if n:
+            lib().vrw_find_lights(self.h, ptr(out), n)
+        return out
+
+    def setup_light_buffer(self, lights: np.ndarray, pos: int = 0):
+        L = np.ascontiguousarray(lights, dtype=LIGHT_DTYPE)
+        lib().vrw_setup_light_buffer(self.h, ptr(L) if len(L) else None, len(L), pos)
+
+    def add_light(self, x, y, z, size, c):
+        lib().vrw_add_light(self.h, x, y, z, size, c)
+
+    def remove_light(self, x, y, z):
+        lib().vrw_remove_light(self.h, x, y, z)
+
+    # host data
+    def grid(self) -> np.ndarray:
+        p = lib().vrw_grid(self.h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), (_lib.GRIDSIZE,)).copy()
+
+    def high_water(self) -> int:
+        return lib().vrw_brick_high_water(self.h)
+
+    def bricks(self) -> np.ndarray:
+        n = self.high_water() * _lib.BRICKSIZE
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        p = lib().vrw_bricks(self.h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), (n,)).copy()
+
+    def zeroes(self) -> np.ndarray:
+        p = lib().vrw_zeroes(self.h)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), (self.cfg.brick_capacity,)).copy()
+
+    def dirty(self) -> np.ndarray:
+        n = lib().vrw_dirty_bricks(self.h, None, 0)
+        out = np.zeros(n, np.uint32)
+        if n:
+            lib().vrw_dirty_bricks(self.h, ptr(out), n)
+        return out
+
+    # camera / params
+    def look_at(self, pos, target):
+        P = np.asarray(pos, np.float32)
+        T = np.asarray(target, np.float32)
+        lib().vrw_look_at(self.h, ptr(P), ptr(T))
+
+    def set_focal(self, f: float):
+        lib().vrw_set_focal(self.h, C.c_float(f))
+
+    @property
+    def params(self) -> VrRenderParams:
+        return C.cast(lib().vrw_params(self.h), C.POINTER(VrRenderParams)).contents
+
+    def set_next_r0(self, r0: int):
+        lib().vrw_set_next_r0(self.h, r0 & 0xFFFFFFFF)
+
+    # per frame
+    def render(self):
+        self._chk(lib().vrw_render(self.h), "render")
+
+    def prepare_frame(self) -> VrRenderParams:
+        """Set up the next frame's params (no launch); returns a copy."""
+        lib().vrw_prepare_frame(self.h)
+        out = VrRenderParams()
+        C.memmove(C.byref(out), C.byref(self.params), C.sizeof(VrRenderParams))
+        return out
+
+    def finish_frame(self):
+        lib().vrw_finish_frame(self.h)
+
+    def commit(self):
+        self._chk(lib().vrw_commit(self.h), "commit")
+
+    def force_sync(self):
+        self._chk(lib().vrw_force_sync(self.h), "force_sync")
+
+    @property
+    def renderer(self) -> Renderer:
+        """The device context of this world as a Renderer (not owned)."""
+        if self._renderer is None:
+            ctx = lib().vrw_context(self.h)
+            if not ctx:
+                raise _lib.VrError(_lib.VR_ERR_STATE, "no device context: " + (lib().vrw_last_error(self.h) or b"").decode())
+            r = Renderer.__new__(Renderer)
+            r.cfg, r.h, r.n = self.cfg, ctx, self.cfg.width * self.cfg.height
+            r.close = lambda: None  # owned by the World
+            self._renderer = r
+        return self._renderer
+
+    # scenes
+    def scene_terrain(self, size: int, scale: int, seed: int):
+        self._chk(lib().vrw_scene_terrain(self.h, size, scale, seed), "scene_terrain")
+
+    def scene_boxes(self, n, emin, emax, size, scale, terrain_seed, seed):
+        self._chk(lib().vrw_scene_boxes(self.h, n, emin, emax, size, scale, terrain_seed, seed), "scene_boxes")
+
+    def scene_lights_above(self, n, size, scale, terrain_seed, seed, hmin, hmax, mmin, mmax):
+        self._chk(lib().vrw_scene_lights_above(self.h, n, size, scale, terrain_seed, seed, hmin, hmax, mmin, mmax),
+                  "scene_lights_above")
+
+    def scene_lights_surface(self, n, size, seed, mmin, mmax):
+        self._chk(lib().vrw_scene_lights_surface(self.h, n, size, seed, mmin, mmax), "scene_lights_surface")
+
+
+def terrain_height(x: float, z: float, scale: int, seed: int) -> float:
+    return lib().vrw_terrain_height(C.c_float(x), C.c_float(z), scale, seed)
+
+
+def camera_path(kind: int, seed: int, frame: int, nframes: int, size: int, scale: int, terrain_seed: int):
+    pos = np.zeros(3, np.float32)
+    tgt = np.zeros(3, np.float32)
+    check(lib().vrw_camera_path(kind, seed, frame, nframes, size, scale, terrain_seed, ptr(pos), ptr(tgt)), None,
+          "camera_path")
+    return pos, tgt

@@ -1,0 +1,76 @@
+"""Build the MI355X Voxel ReSTIR library (smallproject_amd/lib/libvoxelrestir.so).
+
+The HIP sources are compiled for gfx950 with hipcc; the host C++ World and
+scene generators are compiled by hipcc's host compiler into the same shared
+object.  Every translation unit is built with -ffp-contract=off: the kernels
+and the host-side light/camera set-up follow a fixed IEEE-754 operation order.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+LIBDIR = PKG / "lib"
+LIBNAME = "libvoxelrestir.so"
+
+HIP_SOURCES = ["vr_kernels.hip", "vr_runtime.hip"]
+CXX_SOURCES = ["world.cpp", "world_capi.cpp", "scenes.cpp"]
+
+COMMON = ["-O3", "-std=c++17", "-ffp-contract=off", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}"]
+# Packed FP32 (v_pk_*_f32) instructions are disabled for all device code.
+HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the MI355X library cannot be built")
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    if out.strip():
+        print(out, file=sys.stderr)
+
+
+def build(verbose: bool = False) -> Path:
+    """Compile every source and link the shared library; returns its path."""
+    cc = hipcc()
+    objdir = PKG / "_objs"
+    objdir.mkdir(exist_ok=True)
+    LIBDIR.mkdir(exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = objdir / (src + ".o")
+        cmd = [cc, "-x", "hip", *COMMON, *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(str(obj))
+    for src in CXX_SOURCES:
+        obj = objdir / (src + ".o")
+        cmd = [cc, "-x", "c++", *COMMON, "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(str(obj))
+    out = LIBDIR / LIBNAME
+    tmp = LIBDIR / (LIBNAME + ".tmp")
+    _run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,93 @@
+"""Benchmark configurations: seeded scenes, cameras and ReSTIR settings.
+
+Each configuration names its world generator, light placement, camera path
+and render settings.  `build_world` constructs the host World (CPU only);
+`World.force_sync()` then mirrors it to the device.
+
+  c1_point_128   plain biased Voxel ReSTIR, point lights, 128^3 heightfield
+                 embedded in the 1024^3 map, 64 lights, 640x360, 8 frames
+  c2_area_1024   plain Voxel ReSTIR, area lights (per-face solid-angle
+                 sampling), 1024^3 heightfield x8 + 2000 boxes, 4096 emissive
+                 surface voxels, 1920x1080, 60-frame spline path
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+from . import _lib
+from .api import World, camera_path, default_config, focal_for_fov, log_lut
+
+
+@dataclass
+class SceneConfig:
+    name: str
+    width: int
+    height: int
+    point: bool
+    fov: float
+    frames: int
+    world_size: int          # active region edge, voxels
+    scale: int               # heightfield scale
+    terrain_seed: int
+    boxes: int = 0
+    box_edge: tuple = (4, 32)
+    box_seed: int = 0
+    lights: int = 0
+    light_seed: int = 0
+    light_mode: str = "above"     # "above" (hmin..hmax over terrain) or "surface"
+    light_height: tuple = (2, 20)
+    emit_range: tuple = (1, 10)   # emit byte range
+    lut: tuple | None = None      # (lo, hi) log LUT or None for EmitStrength = byte
+    camera_kind: int = 0
+    camera_seed: int = 0
+    candidates: int = 32
+    taps: int = 5
+    radius: int = 30
+    temporal_clamp: int = 20
+    quirks: int = _lib.QUIRKS_REFERENCE
+    source: int = _lib.SOURCE_UNIFORM
+    extra: dict = field(default_factory=dict)
+
+    def config(self, width: int | None = None, height: int | None = None):
+        lut = None if self.lut is None else log_lut(*self.lut)
+        return default_config(width or self.width, height or self.height, point=self.point, quirks=self.quirks,
+                              source=self.source, lut=lut)
+
+    def camera(self, frame: int):
+        return camera_path(self.camera_kind, self.camera_seed, frame, self.frames, self.world_size, self.scale,
+                           self.terrain_seed)
+
+
+CONFIGS = {
+    "c1_point_128": SceneConfig(
+        name="c1_point_128", width=640, height=360, point=True, fov=60.0, frames=8, world_size=128, scale=1,
+        terrain_seed=1, lights=64, light_seed=2, light_mode="above", light_height=(2, 20), emit_range=(1, 10),
+        camera_kind=0, camera_seed=3),
+    "c2_area_1024": SceneConfig(
+        name="c2_area_1024", width=1920, height=1080, point=False, fov=60.0, frames=60, world_size=1024, scale=8,
+        terrain_seed=1, boxes=2000, box_edge=(4, 32), box_seed=4, lights=4096, light_seed=5,
+        light_mode="surface", emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=6),
+}
+
+
+def build_world(sc: SceneConfig | str, width: int | None = None, height: int | None = None) -> World:
+    """Build the host World of a configuration (no GPU needed)."""
+    if isinstance(sc, str):
+        sc = CONFIGS[sc]
+    w = World(sc.config(width, height))
+    w.scene_terrain(sc.world_size, sc.scale, sc.terrain_seed)
+    if sc.boxes:
+        w.scene_boxes(sc.boxes, sc.box_edge[0], sc.box_edge[1], sc.world_size, sc.scale, sc.terrain_seed, sc.box_seed)
+    w.optimize()
+    if sc.lights:
+        if sc.light_mode == "above":
+            w.scene_lights_above(sc.lights, sc.world_size, sc.scale, sc.terrain_seed, sc.light_seed,
+                                 sc.light_height[0], sc.light_height[1], sc.emit_range[0], sc.emit_range[1])
+        else:
+            w.scene_lights_surface(sc.lights, sc.world_size, sc.light_seed, sc.emit_range[0], sc.emit_range[1])
+    w.set_focal(focal_for_fov(sc.fov))
+    p = w.params
+    p.numberOfCandidates, p.spatialTaps, p.spatialRadius = sc.candidates, sc.taps, sc.radius
+    p.numberOfMaxTemporalImportance = sc.temporal_clamp
+    p.skyDomeSampling = 0
+    return w

@@ -1,0 +1,542 @@
+// vr_kernels.hip - gfx950 kernels of the Voxel ReSTIR frame.
+//
+// One frame = four launches on the context stream, each one thread per pixel
+// with 256-thread workgroups covering a 16x16 pixel tile (four 8x8 wave64
+// tiles, so the rays of a wave are spatially coherent):
+//   k_primary   renderAlbedo                cl/ris.cl:295-389
+//   k_initial   perPixelInitialSampling     cl/ris.cl:20-124
+//   k_spatial   perPixelSpatialResampling   cl/ris.cl:126-151 + spatialSampling.cl
+//   k_shade     renderRIS / render_di_ris   cl/ris.cl:155-293, 391-413
+// Per-pixel state lives in structure-of-arrays planes (12 B of hit data and
+// 36 B per reservoir) so every pass streams coalesced 4/16-byte words.
+#include "vr_kernels.h"
+
+namespace vr {
+
+__device__ __forceinline__ void tile_pixel(int& x, int& y)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    x = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
+    y = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
+}
+
+__device__ __forceinline__ Res load_res(const ResPlanes& r, size_t i)
+{
+    const float4 a = r.a[i], b = r.b[i];
+    const uint32_t m = r.m[i];
+    Res o;
+    o.wsum = a.x, o.W = a.y, o.pHat = a.z, o.invPdf = a.w;
+    o.pos = f3{b.x, b.y, b.z};
+    o.light = __float_as_uint(b.w);
+    o.M = m & 0xffffffu;
+    o.face = m >> 24;
+    return o;
+}
+
+__device__ __forceinline__ void store_res(const ResPlanes& r, size_t i, const Res& o)
+{
+    r.a[i] = make_float4(o.wsum, o.W, o.pHat, o.invPdf);
+    r.b[i] = make_float4(o.pos.x, o.pos.y, o.pos.z, __uint_as_float(o.light));
+    r.m[i] = (o.M & 0xffffffu) | (o.face << 24);
+}
+
+// GridCoordinatesFromHit + Uint3CubeEquals (cl/tools.cl:452-472) behind
+// isLightOccluded (cl/ris.cl:1-17).  A shadow ray that finds no opaque voxel
+// within its step budget counts as occluded.
+__device__ __forceinline__ bool light_occluded(const WorldView& w, uint32_t lpos, uint32_t lsize, f3 spo, f3 L,
+                                               int steps)
+{
+    float dist = 0.0f;
+    uint32_t side = 0;
+    const uint32_t v = trace<kTraceOpaque>(w, spo, L, steps, 0u, dist, side);
+    if (v == 0u) return true;
+    const f3 N = voxel_normal(side, L);
+    const uint32_t hx = f2u((L.x * dist + spo.x) - N.x * 0.1f);
+    const uint32_t hy = f2u((L.y * dist + spo.y) - N.y * 0.1f);
+    const uint32_t hz = f2u((L.z * dist + spo.z) - N.z * 0.1f);
+    const uint32_t ly = lpos >> 20, lz = (lpos >> 10) & 1023u, lx = lpos & 1023u;
+    const bool inside = lx <= hx && lx + lsize > hx && ly <= hy && ly + lsize > hy && lz <= hz && lz + lsize > hz;
+    return !inside;
+}
+
+// same-face test of temporal / spatial reuse; hit words carry side (bits
+// 24-25) and the sign of D[side] (bit 26)
+__device__ __forceinline__ bool same_face(uint32_t a, uint32_t b, bool axis_only)
+{
+    const uint32_t m = axis_only ? (3u << 24) : (7u << 24);
+    return ((a ^ b) & m) == 0u;
+}
+
+// RandomPointOnVoxel (cl/tools.cl:499-577): 12 draws for the six faces,
+// one for the face CDF.
+__device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t& seed, f3& pos, float& invPdf,
+                                               uint32_t& face)
+{
+    float solid[6];
+    f3 pts[6];
+    float total = 0.0f;
+    const float hs = 0.5f * size;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const f3 N = face_normal((uint32_t)i);
+        const int ti = i < 3 ? (i + 1) % 3 : 3 + (i - 2) % 3;   // 1,2,0,4,5,3
+        const int bi = i < 3 ? (i + 2) % 3 : 3 + (i - 1) % 3;   // 2,0,1,5,3,4
+        const f3 T = face_normal((uint32_t)ti), Bn = face_normal((uint32_t)bi);
+        const float r0 = rnd(seed), r1 = rnd(seed);
+        const float s0 = r0 * size, s1 = r1 * size;
+        const f3 pt = f3{c.x + hs * N.x - hs * T.x - hs * Bn.x + s0 * T.x + s1 * Bn.x,
+                         c.y + hs * N.y - hs * T.y - hs * Bn.y + s0 * T.y + s1 * Bn.y,
+                         c.z + hs * N.z - hs * T.z - hs * Bn.z + s0 * T.z + s1 * Bn.z};
+        const f3 R = sub(pt, sp);
+        const f3 L = normalize(R);
+        const float d2 = fmaxf(1e-5f, dot(R, R));
+        const float ndl = dot(N, neg(L));
+        const float sa = ndl * (size * size) / d2;
+        pts[i] = pt;
+        solid[i] = fmaxf(0.0f, sa);
+        total += fmaxf(0.0f, sa);
+    }
+    const float r2 = rnd(seed);
+    uint32_t index = 0;
+    float pdf = solid[0] / total;
+    float cdf = pdf;
+    if (cdf < r2) {
+#pragma unroll
+        for (int i = 1; i < 6; i++) {
+            pdf = solid[i] / total;
+            cdf = cdf + pdf;
+            index = (uint32_t)i;
+            if (cdf >= r2) break;
+        }
+    }
+    float wgt = 1.0f / pdf;
+    if (isinf(wgt) || isnan(wgt)) wgt = 0.0f;
+    invPdf = wgt;
+    face = index;
+    pos = pts[0];
+#pragma unroll
+    for (int i = 1; i < 6; i++)
+        if (index == (uint32_t)i) pos = pts[i];
+}
+
+__device__ __forceinline__ uint32_t pick_light(const LightView& L, uint32_t source, uint32_t& seed, float& p)
+{
+    // GenerateLightCandidate + getRandomIndex (cl/tools.cl:423-426, 593-598)
+    const uint32_t N = L.n;
+    const float u = rnd(seed);
+    int32_t i = f2i(u * (float)N);
+    i = i < 0 ? 0 : (i > (int32_t)N - 1 ? (int32_t)N - 1 : i);
+    if (source == 1u) {
+        const float fr = u * (float)N - (float)i;
+        const uint32_t k = fr < L.prob[i] ? (uint32_t)i : L.alias[i];
+        p = L.pmf[k];
+        return k;
+    }
+    p = 1.0f / (float)N;
+    return (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------
+// k_primary: renderAlbedo (cl/ris.cl:295-389)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_primary(FrameArgs a)
+{
+    int x, y;
+    tile_pixel(x, y);
+    if (x >= (int)a.W || y >= (int)a.H) return;
+    const size_t idx = (size_t)y * a.W + x;
+    const uint32_t seed = wang_hash((uint32_t)x * 171u + (uint32_t)y * 1773u + a.R0);
+    const f3 D = camera_ray(a.cam, x, y);
+    uint32_t side = 0;
+    float dist = 0.0f;
+    uint32_t voxel = trace<kTraceAll>(a.w, a.cam.E, D, 999999, 0u, dist, side);
+    float total = 0.0f;
+    uint32_t enc;
+    if (voxel != 0u && (voxel & 0xf000u) == 0xf000u) {
+        // opaque first hit: the translucency loop is skipped
+        // (remainingVisibility = 0); re-encode as FromFloatRGBA does
+        const f3 c = to_rgb(voxel);
+        total = dist;
+        enc = (((voxel >> 16) & 255u) << 16) + (f2u(1.0f * 15.0f) << 12) + (f2u(c.x * 1.0f * 15.0f) << 8) +
+              (f2u(c.y * 1.0f * 15.0f) << 4) + f2u(c.z * 1.0f * 15.0f);
+    } else {
+        float alpha = alphaf(voxel);
+        f3 c = to_rgb(voxel);
+        f3 out = f3{c.x * alpha, c.y * alpha, c.z * alpha};
+        int iteration = 0;
+        float remaining = 1.0f - alphaf(voxel);
+        uint32_t prevSide = 0, prevVoxel = 0;
+        while (prevVoxel != voxel && remaining > 0.0f) {
+            prevVoxel = voxel;
+            prevSide = side;
+            total += dist + 0.01f;
+            const f3 O = f3{a.cam.E.x + total * D.x, a.cam.E.y + total * D.y, a.cam.E.z + total * D.z};
+            voxel = trace<kTraceThrough>(a.w, O, D, 999999, voxel, dist, side);
+            c = to_rgb(voxel);
+            alpha = alphaf(voxel);
+            out.x += remaining * c.x * alpha, out.y += remaining * c.y * alpha, out.z += remaining * c.z * alpha;
+            remaining = remaining * (1 - alpha);
+            if (++iteration == 25) remaining = 0.0f;
+            if (dist <= 0.001f) break;
+        }
+        if (voxel != 0u) total += dist;
+        else side = prevSide;
+        enc = (((voxel >> 16) & 255u) << 16) + (f2u((1.0f - remaining) * 15.0f) << 12) + (f2u(out.x * 15.0f) << 8) +
+              (f2u(out.y * 15.0f) << 4) + f2u(out.z * 15.0f);
+    }
+    const float ds = side == 0 ? D.x : (side == 1 ? D.y : D.z);
+    a.cur.hitw[idx] = enc | (side << 24) | ((ds > 0 ? 1u : 0u) << 26);
+    a.cur.dist[idx] = total;
+    a.cur.seed[idx] = seed;
+}
+
+// shading point set-up shared by the reuse passes
+struct Shading {
+    f3 D, N, brdf, sp;
+};
+
+__device__ __forceinline__ Shading shading_at(const FrameArgs& a, int x, int y, uint32_t hitw, float dist)
+{
+    Shading s;
+    s.D = camera_ray(a.cam, x, y);
+    s.N = voxel_normal((hitw >> 24) & 3u, s.D);
+    const f3 c = to_rgb(hitw);
+    s.brdf = f3{c.x * kInvPi, c.y * kInvPi, c.z * kInvPi};
+    s.sp = f3{s.D.x * dist + a.cam.E.x, s.D.y * dist + a.cam.E.y, s.D.z * dist + a.cam.E.z};
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// k_initial: candidates + visibility + temporal reuse (cl/ris.cl:20-124)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_initial(FrameArgs a)
+{
+    int x, y;
+    tile_pixel(x, y);
+    if (x >= (int)a.W || y >= (int)a.H) return;
+    const size_t idx = (size_t)y * a.W + x;
+    const uint32_t hitw = a.cur.hitw[idx];
+    const float dist = a.cur.dist[idx];
+    Res res;
+    res.wsum = 0.0f, res.W = 0.0f, res.pHat = 0.0f, res.invPdf = 0.0f;
+    res.pos = f3{0.0f, 0.0f, 0.0f}, res.light = 0u, res.M = 0u, res.face = 7u;
+    if ((hitw & 0xffffffu) != 0u) {
+        uint32_t seed = a.cur.seed[idx];
+        const Shading s = shading_at(a, x, y, hitw, dist);
+        const f3 spo = f3{s.sp.x + 0.1f * s.N.x, s.sp.y + 0.1f * s.N.y, s.sp.z + 0.1f * s.N.z};
+        const uint32_t nL = a.L.n;
+        const uint32_t nC = nL < a.nCand ? nL : a.nCand;
+        for (uint32_t i = 0; i < nC; i++) {
+            float pr;
+            const uint32_t li = pick_light(a.L, a.source, seed, pr);
+            const float4 la = a.L.a[li], lb = a.L.b[li];
+            f3 pos;
+            float invPdf;
+            uint32_t face;
+            if (a.point) {
+                pos = f3{la.x, la.y, la.z}, invPdf = 1.0f, face = 7u;
+            } else {
+                point_on_voxel(f3{la.x, la.y, la.z}, la.w, spo, seed, pos, invPdf, face);
+            }
+            const float ph = phat(a.point, pos, face_normal(face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) * invPdf;
+            res_update(res, ph / pr, ph, li, rnd(seed), pos, invPdf, face);
+        }
+        res_adjust(res, res.pHat);
+        if (res.wsum != 0.0f) {
+            // with wsum == 0 the discard would leave W = wsum = 0 unchanged
+            const float4 lb = a.L.b[res.light];
+            const float4 la = a.L.a[res.light];
+            const f3 L = normalize(sub(res.pos, spo));
+            if (light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps))
+                res.W = 0.0f, res.wsum = 0.0f;
+        }
+        if (a.restirtemporalframe > 0u && a.temporal) {
+            // ReprojectWorldPoint (cl/tools.cl:479-488)
+            const float dl = dot(s.sp, f3{a.Nl.x, a.Nl.y, a.Nl.z}) - a.Nl.w;
+            const float dr = dot(s.sp, f3{a.Nr.x, a.Nr.y, a.Nr.z}) - a.Nr.w;
+            const float dt = dot(s.sp, f3{a.Nt.x, a.Nt.y, a.Nt.z}) - a.Nt.w;
+            const float db = dot(s.sp, f3{a.Nb.x, a.Nb.y, a.Nb.z}) - a.Nb.w;
+            const int px = f2i(roundf((float)a.W * (dl / (dl + dr))));
+            const int py = f2i(roundf((float)a.H * (dt / (dt + db))));
+            if (px >= 0 && py >= 0 && px < (int)a.W && py < (int)a.H) {
+                const size_t pi = (size_t)py * a.W + px;
+                const uint32_t ph = a.prevHitw[pi];
+                const float pd = a.prevDist[pi];
+                if (same_face(ph, hitw, a.axisOnlyFace) && fabsf(dist - pd) <= 0.1f * dist) {
+                    Res prev = load_res(a.A, pi);
+                    const uint32_t cap = a.maxTemporal * nC;
+                    const uint32_t pm = cap < prev.M ? cap : prev.M;
+                    if (pm > 0u && prev.light < nL) {
+                        const float4 pa = a.L.a[prev.light], pb = a.L.b[prev.light];
+                        const float pph = phat(a.point, prev.pos, face_normal(prev.face), s.sp, s.N, s.brdf,
+                                               f3{pb.x, pb.y, pb.z}, pa.w) * prev.invPdf;
+                        res_reweigh(prev, pph, pm);
+                        res_combine(res, prev, rnd(seed));
+                        const float4 ra = a.L.a[res.light], rb = a.L.b[res.light];
+                        const float rph = phat(a.point, res.pos, face_normal(res.face), s.sp, s.N, s.brdf,
+                                               f3{rb.x, rb.y, rb.z}, ra.w) * res.invPdf;
+                        res_adjust(res, rph);
+                    }
+                }
+            }
+        }
+        a.cur.seed[idx] = seed;
+    }
+    store_res(a.B, idx, res);
+}
+
+// ---------------------------------------------------------------------------
+// k_spatial: spatial reuse over a disc (cl/spatialSampling.cl:2-67)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
+{
+    int x, y;
+    tile_pixel(x, y);
+    if (x >= (int)a.W || y >= (int)a.H) return;
+    const size_t idx = (size_t)y * a.W + x;
+    Res res = load_res(a.B, idx);
+    const uint32_t hitw = a.cur.hitw[idx];
+    if ((hitw & 0xffffffu) != 0u && a.spatial) {
+        const float dist = a.cur.dist[idx];
+        uint32_t seed = a.cur.seed[idx];
+        const Shading s = shading_at(a, x, y, hitw, dist);
+        const float rad = (float)(a.spatialRadius - 1u);
+        for (uint32_t i = 0; i < a.spatialTaps; i++) {
+            const float angle = rnd(seed) * kTwoPi;
+            const float radius = rad * sqrtf(rnd(seed)) + 1.0f;
+            float sn, cs;
+            sincos_det(angle, sn, cs);
+            const int nx = x + f2i(cs * radius), ny = y + f2i(sn * radius);
+            if ((nx == x && ny == y) || nx < 0 || ny < 0 || nx >= (int)a.W || ny >= (int)a.H) continue;
+            const size_t ni = (size_t)ny * a.W + nx;
+            const uint32_t nm = a.B.m[ni];
+            if ((nm & 0xffffffu) == 0u) continue;
+            const uint32_t nh = a.cur.hitw[ni];
+            const float nd = a.cur.dist[ni];
+            if (!same_face(nh, hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist)) continue;
+            Res nr = load_res(a.B, ni);
+            if (nr.light >= a.L.n) continue;
+            const float4 la = a.L.a[nr.light], lb = a.L.b[nr.light];
+            const float ph = phat(a.point, nr.pos, face_normal(nr.face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) *
+                             nr.invPdf;
+            res_reweigh(nr, ph, nr.M);
+            res_combine(res, nr, rnd(seed));
+        }
+        res_adjust(res, res.pHat);
+        a.cur.seed[idx] = seed;
+    }
+    store_res(a.A, idx, res);
+}
+
+// ---------------------------------------------------------------------------
+// k_shade: final shading (cl/ris.cl:155-293 with sky dome and editor off)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_shade(FrameArgs a)
+{
+    int x, y;
+    tile_pixel(x, y);
+    if (x >= (int)a.W || y >= (int)a.H) return;
+    const size_t idx = (size_t)y * a.W + x;
+    const uint32_t hitw = a.cur.hitw[idx];
+    const uint32_t voxel = hitw & 0xffffffu;
+    const float dist = a.cur.dist[idx];
+    f3 color = f3{0.0f, 0.0f, 0.0f};
+    if (voxel != 0u) {
+        const Shading s = shading_at(a, x, y, hitw, dist);
+        if (is_emitter(voxel)) {
+            const f3 c = to_rgb(voxel);
+            const float es = a.emitLut[(voxel >> 16) & 255u];
+            f3 e = f3{c.x * es, c.y * es, c.z * es};
+            if (a.gfxexp) e = f3{e.x * kInvPi, e.y * kInvPi, e.z * kInvPi};
+            color = f3{color.x + e.x, color.y + e.y, color.z + e.z};
+        }
+        if (a.gfxexp || !is_emitter(voxel)) {
+            const float4 ra = a.A.a[idx];
+            const uint32_t m = a.A.m[idx];
+            if (ra.y > 0.0f && (m & 0xffffffu) > 0u) {
+                const Res res = load_res(a.A, idx);
+                if (res.light < a.L.n) {
+                    const f3 spo = f3{s.sp.x + 0.1f * s.N.x, s.sp.y + 0.1f * s.N.y, s.sp.z + 0.1f * s.N.z};
+                    const float4 la = a.L.a[res.light], lb = a.L.b[res.light];
+                    const f3 L = normalize(sub(res.pos, spo));
+                    if (!light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps)) {
+                        const f3 inc = phat_full(a.point, res.pos, face_normal(res.face), s.sp, s.N, s.brdf,
+                                                 f3{lb.x, lb.y, lb.z}, la.w);
+                        color.x += inc.x * res.W * res.invPdf;
+                        color.y += inc.y * res.W * res.invPdf;
+                        color.z += inc.z * res.W * res.invPdf;
+                    } else {
+                        a.A.a[idx] = make_float4(0.0f, 0.0f, ra.z, ra.w);
+                    }
+                }
+            }
+        }
+        if (a.doubleRadiance) {
+            const float al = alphaf(voxel);
+            color.x += al * color.x + (1 - al) * 0.0f;
+            color.y += al * color.y + (1 - al) * 0.0f;
+            color.z += al * color.z + (1 - al) * 0.0f;
+        }
+    }
+    a.frame[idx] = make_float4(color.x, color.y, color.z, dist);
+}
+
+// ---------------------------------------------------------------------------
+// support kernels
+// ---------------------------------------------------------------------------
+
+// commit (cl/kernels.cl:448-477): one 128-thread group per staged brick,
+// 16 bytes per thread
+__global__ __launch_bounds__(128) void k_commit(const uint32_t* __restrict__ ids, const uint4* __restrict__ src,
+                                                uint4* __restrict__ pool, uint32_t n, uint32_t cap)
+{
+    const uint32_t b = blockIdx.x;
+    if (b >= n) return;
+    const uint32_t id = ids[b];
+    if (id >= cap) return;
+    pool[(size_t)id * 128u + threadIdx.x] = src[(size_t)b * 128u + threadIdx.x];
+}
+
+// updateUberGrid (cl/kernels.cl:498-513)
+__global__ __launch_bounds__(256) void k_uber(const uint32_t* __restrict__ grid, uint8_t* __restrict__ uber)
+{
+    const uint32_t task = blockIdx.x * 256u + threadIdx.x;
+    if (task >= 32768u) return;
+    const uint32_t x = task & 31u, z = (task >> 5) & 31u, y = task >> 10;
+    uint32_t any = 0u;
+    for (uint32_t b = 0; b < 4; b++)
+        for (uint32_t c = 0; c < 4; c++) {
+            const uint4 row = *reinterpret_cast<const uint4*>(grid + (x * 4) + (z * 4 + c) * 128u + (y * 4 + b) * 16384u);
+            any |= row.x | row.y | row.z | row.w;
+        }
+    uber[x + z * 32u + y * 1024u] = any ? 1 : 0;
+}
+
+// traceBatch (cl/kernels.cl:395-417) returning voxel, dist, side
+__global__ __launch_bounds__(256) void k_trace_batch(WorldView w, const float* __restrict__ O, const float* __restrict__ D,
+                                                     uint32_t n, uint32_t kind, int32_t steps, uint32_t* voxel,
+                                                     float* dist, uint32_t* side)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const f3 A = f3{O[3 * i], O[3 * i + 1], O[3 * i + 2]};
+    const f3 B = f3{D[3 * i], D[3 * i + 1], D[3 * i + 2]};
+    float d = 0.0f;
+    uint32_t s = 0;
+    const uint32_t v = kind == 1u ? trace<kTraceOpaque>(w, A, B, steps, 0u, d, s)
+                                  : trace<kTraceAll>(w, A, B, steps, 0u, d, s);
+    voxel[i] = v;
+    dist[i] = v ? d : 0.0f;
+    side[i] = v ? s : 0u;
+}
+
+// ground truth: render_di_gfxexp (cl/kernels.cl:177-241) with
+// UniformSampleHemisphere (cl/tools.cl:31-52), binary64 accumulation
+__global__ __launch_bounds__(256) void k_ground_truth(FrameArgs a, uint32_t spp, uint32_t frame0, double* accum)
+{
+    int x, y;
+    tile_pixel(x, y);
+    if (x >= (int)a.W || y >= (int)a.H) return;
+    const size_t idx = (size_t)y * a.W + x;
+    const f3 D = camera_ray(a.cam, x, y);
+    float dist = 0.0f;
+    uint32_t side = 0;
+    const uint32_t voxel = trace<kTraceAll>(a.w, a.cam.E, D, 999999, 0u, dist, side);
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    if (voxel != 0u) {
+        const f3 base = to_rgb(voxel);
+        const float es = a.emitLut[(voxel >> 16) & 255u];
+        const f3 contrib = f3{base.x * es * kInvPi, base.y * es * kInvPi, base.z * es * kInvPi};
+        const f3 brdf = f3{kInvPi * base.x, kInvPi * base.y, kInvPi * base.z};
+        const f3 I = f3{a.cam.E.x + D.x * dist, a.cam.E.y + D.y * dist, a.cam.E.z + D.z * dist};
+        const f3 N = voxel_normal(side, D);
+        f3 Nt;
+        if (fabsf(N.x) > fabsf(N.y)) {
+            const float l = sqrtf(N.x * N.x + N.z * N.z);
+            Nt = f3{N.z / l, 0.0f / l, -N.x / l};
+        } else {
+            const float l = sqrtf(N.y * N.y + N.z * N.z);
+            Nt = f3{0.0f / l, -N.z / l, N.y / l};
+        }
+        const f3 Nb = f3{N.y * Nt.z - N.z * Nt.y, N.z * Nt.x - N.x * Nt.z, N.x * Nt.y - N.y * Nt.x};
+        const f3 O2 = f3{I.x + 0.1f * N.x, I.y + 0.1f * N.y, I.z + 0.1f * N.z};
+        uint32_t r0state = a.R0;
+        for (uint32_t f = 0; f < spp; f++) {
+            const uint32_t R0 = xorshift(r0state);
+            uint32_t seed = wang_hash((uint32_t)x * 171u + (uint32_t)y * 1773u + R0);
+            const uint32_t frame = (frame0 + f) & 255u;
+            const float inv16 = 1.0f / 16.0f;
+            const float r0 = (float)(int)(frame & 15u) * inv16 + rnd(seed) * inv16;
+            const float r1 = (float)(int)(frame >> 4) * inv16 + rnd(seed) * inv16;
+            const float sinTheta = sqrtf(1 - r0 * r0);
+            float sn, cs;
+            sincos_det(kTwoPi * r1, sn, cs);
+            const float xs = sinTheta * cs, zs = sinTheta * sn;
+            const f3 R = f3{xs * Nb.x + r0 * N.x + zs * Nt.x, xs * Nb.y + r0 * N.y + zs * Nt.y,
+                            xs * Nb.z + r0 * N.z + zs * Nt.z};
+            const float NdotL = fmaxf(0.0f, dot(N, R));
+            float d2 = 0.0f;
+            uint32_t s2 = 0;
+            const uint32_t v2 = trace<kTraceAll>(a.w, O2, R, 128, 0u, d2, s2);
+            const f3 sl0 = to_rgb(v2);
+            const float es2 = a.emitLut[(v2 >> 16) & 255u];
+            const f3 sl = f3{sl0.x * es2 * kInvPi, sl0.y * es2 * kInvPi, sl0.z * es2 * kInvPi};
+            acc0 += (double)((sl.x * kTwoPi * NdotL) * brdf.x + contrib.x);
+            acc1 += (double)((sl.y * kTwoPi * NdotL) * brdf.y + contrib.y);
+            acc2 += (double)((sl.z * kTwoPi * NdotL) * brdf.z + contrib.z);
+        }
+    }
+    accum[idx * 3 + 0] += acc0;
+    accum[idx * 3 + 1] += acc1;
+    accum[idx * 3 + 2] += acc2;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 frame_grid(uint32_t W, uint32_t H) { return dim3((W + 15) / 16, (H + 15) / 16); }
+
+hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s)
+{
+    const dim3 g = frame_grid(a.W, a.H);
+    switch (stage) {
+    case 0: hipLaunchKernelGGL(k_primary, g, dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL(k_initial, g, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_spatial, g, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_shade, g, dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_commit(const uint32_t* ids, const uint32_t* src, uint32_t* pool, uint32_t n, uint32_t cap,
+                         hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_commit, dim3(n), dim3(128), 0, s, ids, reinterpret_cast<const uint4*>(src),
+                       reinterpret_cast<uint4*>(pool), n, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_uber(const uint32_t* grid, uint8_t* uber, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_uber, dim3(128), dim3(256), 0, s, grid, uber);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_batch(const WorldView& w, const float* O, const float* D, uint32_t n, uint32_t kind,
+                              int32_t steps, uint32_t* voxel, float* dist, uint32_t* side, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trace_batch, dim3((n + 255) / 256), dim3(256), 0, s, w, O, D, n, kind, steps, voxel, dist,
+                       side);
+    return hipGetLastError();
+}
+
+hipError_t launch_ground_truth(const FrameArgs& a, uint32_t spp, uint32_t frame0, double* accum, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ground_truth, frame_grid(a.W, a.H), dim3(256), 0, s, a, spp, frame0, accum);
+    return hipGetLastError();
+}
+
+}  // namespace vr

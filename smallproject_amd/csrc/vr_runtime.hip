@@ -1,0 +1,541 @@
+// vr_runtime.hip - C ABI of the render path (include/vr_api.h): context,
+// device buffers, commits, light upload, per-frame launches and read-backs.
+// Replaces the reference's OpenCL runtime (template/template.cpp:1266-1754)
+// and the device half of World::Render / World::Commit
+// (template/world.cpp:2127-2483).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vr_api.h"
+#include "vr_kernels.h"
+
+using namespace vr;
+
+static constexpr int kRing = 64;  // frames whose stage events are kept
+
+struct vr_ctx {
+    vr_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // stage boundary events of the last kRing frames
+    hipEvent_t ev[kRing][VR_NUM_STAGES + 1] = {};
+    bool pending[kRing] = {};
+    uint64_t frameNo = 0;
+    int lastSlot = -1;
+    bool profiling = false;
+    double profSum[VR_NUM_STAGES] = {};
+    uint32_t profFrames = 0;
+    // world
+    uint32_t* grid = nullptr;       // [128^3]
+    uint8_t* uber = nullptr;        // [32^3]
+    uint32_t* bricks = nullptr;     // [brick_capacity * 512]
+    uint32_t* stageIds = nullptr;   // [VR_MAXCOMMITS]
+    uint32_t* stageData = nullptr;  // [VR_MAXCOMMITS * 512]
+    // lights
+    float4* la = nullptr;
+    float4* lb = nullptr;
+    float* prob = nullptr;
+    uint32_t* alias = nullptr;
+    float* pmf = nullptr;
+    uint32_t nLights = 0, lightCap = 0;
+    float* emitLut = nullptr;
+    // per-pixel planes
+    uint32_t* hitw[2] = {};
+    float* dist[2] = {};
+    uint32_t* seed[2] = {};
+    float4* ra[2] = {};  // [0] = A (final), [1] = B (initial + temporal)
+    float4* rb[2] = {};
+    uint32_t* rm[2] = {};
+    float4* frame = nullptr;
+    int cur = 1;              // current hit buffer (swapped before each frame)
+    bool committed = false;
+    bool framed = false;
+    std::string err;
+};
+
+namespace {
+
+int fail(vr_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hipfail(vr_ctx* c, hipError_t e, const char* what)
+{
+    return fail(c, VR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hipfail(ctx, e_, #call); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count)
+{
+    return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T) + 16);
+}
+
+void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+// Light record: centre = coords + 0.5*size (cl/tools.cl:606-607);
+// Le = ToFloatRGB(v) * EmitStrength(v) [* 1/pi] (cl/phat.cl:8-12).
+void light_record(const vr_config& cfg, const vr_light& l, float4& a, float4& b)
+{
+    const uint32_t y = l.position / (1024u * 1024u), z = (l.position / 1024u) % 1024u, x = l.position % 1024u;
+    const float hs = 0.5f * (float)l.size;
+    a = make_float4((float)x + hs, (float)y + hs, (float)z + hs, (float)l.size);
+    const float es = cfg.emit_lut[(l.voxel >> 16) & 255u];
+    float r = (float)((l.voxel >> 8) & 15u) * (1.0f / 15.0f);
+    float g = (float)((l.voxel >> 4) & 15u) * (1.0f / 15.0f);
+    float bl = (float)(l.voxel & 15u) * (1.0f / 15.0f);
+    r = r * es, g = g * es, bl = bl * es;
+    if (cfg.gfxexp) r = r * 0.31830988618379067153777f, g = g * 0.31830988618379067153777f,
+                    bl = bl * 0.31830988618379067153777f;
+    uint32_t packed = (y << 20) | (z << 10) | x;
+    float pf;
+    std::memcpy(&pf, &packed, 4);
+    b = make_float4(r, g, bl, pf);
+}
+
+// Power-proportional source: Vose alias table over lightContribution
+// (cl/tools.cl:579-591) reduced to the rgb sum; binary64 arithmetic.
+void power_table(const vr_config& cfg, const vr_light* L, uint32_t n, std::vector<float>& prob,
+                 std::vector<uint32_t>& alias, std::vector<float>& pmf)
+{
+    prob.assign(n, 1.0f), alias.resize(n), pmf.resize(n);
+    std::vector<double> pw(n), sc(n);
+    std::vector<uint32_t> small, large;
+    small.reserve(n), large.reserve(n);
+    const double pi = 3.14159265358979323846;
+    double total = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const double r = (double)((float)((L[i].voxel >> 8) & 15u) * (1.0f / 15.0f));
+        const double g = (double)((float)((L[i].voxel >> 4) & 15u) * (1.0f / 15.0f));
+        const double b = (double)((float)(L[i].voxel & 15u) * (1.0f / 15.0f));
+        const double es = (double)cfg.emit_lut[(L[i].voxel >> 16) & 255u];
+        const double s = (double)L[i].size;
+        const double geo = cfg.light_mode == VR_LIGHTS_POINT ? 4.0 * pi : 6.0 * s * s * pi;
+        pw[i] = (r + g + b) * es * geo;
+        total += pw[i];
+    }
+    if (!(total > 0)) {
+        for (uint32_t i = 0; i < n; i++) pw[i] = 1.0;
+        total = (double)n;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        pmf[i] = (float)(pw[i] / total);
+        sc[i] = pw[i] / total * (double)n;
+        if (sc[i] < 1.0) small.push_back(i);
+        else large.push_back(i);
+        alias[i] = i;
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t s = small.back(), l = large.back();
+        small.pop_back(), large.pop_back();
+        prob[s] = (float)sc[s];
+        alias[s] = l;
+        sc[l] = (sc[l] + sc[s]) - 1.0;
+        if (sc[l] < 1.0) small.push_back(l);
+        else large.push_back(l);
+    }
+    for (uint32_t l : large) prob[l] = 1.0f, alias[l] = l;
+    for (uint32_t s : small) prob[s] = 1.0f, alias[s] = s;
+}
+
+// add a profiled frame's stage times to the running sums (waits for it)
+int harvest(vr_ctx* c, int slot)
+{
+    if (!c->pending[slot]) return VR_OK;
+    c->pending[slot] = false;
+    HIPCHK(c, hipEventSynchronize(c->ev[slot][VR_NUM_STAGES]));
+    for (int s = 0; s < VR_NUM_STAGES; s++) {
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[slot][s], c->ev[slot][s + 1]));
+        c->profSum[s] += ms;
+    }
+    c->profFrames++;
+    return VR_OK;
+}
+
+// non-finite camera data would make every DDA walk its whole step budget
+bool finite_camera(const vr_render_params* p)
+{
+    const float* vals[] = {p->oneOverRes, p->E, p->p0, p->p1, p->p2, p->Nleft, p->Nright, p->Ntop, p->Nbottom};
+    const int counts[] = {2, 3, 3, 3, 3, 4, 4, 4, 4};
+    for (int k = 0; k < 9; k++)
+        for (int i = 0; i < counts[k]; i++)
+            if (!std::isfinite(vals[k][i])) return false;
+    return true;
+}
+
+FrameArgs frame_args(vr_ctx* c, const vr_render_params* p)
+{
+    FrameArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.w = WorldView{c->grid, c->bricks, c->uber};
+    a.L = LightView{c->la, c->lb, c->prob, c->alias, c->pmf, p ? p->numberOfLights : 0u};
+    if (p) {
+        a.cam.ox = p->oneOverRes[0], a.cam.oy = p->oneOverRes[1];
+        a.cam.E = f3{p->E[0], p->E[1], p->E[2]};
+        a.cam.p0 = f3{p->p0[0], p->p0[1], p->p0[2]};
+        a.cam.d10 = f3{p->p1[0] - p->p0[0], p->p1[1] - p->p0[1], p->p1[2] - p->p0[2]};
+        a.cam.d20 = f3{p->p2[0] - p->p0[0], p->p2[1] - p->p0[1], p->p2[2] - p->p0[2]};
+        a.R0 = p->R0;
+        a.restirtemporalframe = p->restirtemporalframe;
+        a.temporal = p->temporal ? 1u : 0u;
+        a.spatial = p->spatial ? 1u : 0u;
+        a.nCand = p->numberOfCandidates;
+        a.maxTemporal = p->numberOfMaxTemporalImportance;
+        a.spatialRadius = p->spatialRadius;
+        a.spatialTaps = p->spatialTaps;
+        a.Nl = make_float4(p->Nleft[0], p->Nleft[1], p->Nleft[2], p->Nleft[3]);
+        a.Nr = make_float4(p->Nright[0], p->Nright[1], p->Nright[2], p->Nright[3]);
+        a.Nt = make_float4(p->Ntop[0], p->Ntop[1], p->Ntop[2], p->Ntop[3]);
+        a.Nb = make_float4(p->Nbottom[0], p->Nbottom[1], p->Nbottom[2], p->Nbottom[3]);
+    }
+    const int cur = c->cur, prev = 1 - c->cur;
+    a.cur = HitPlanes{c->hitw[cur], c->dist[cur], c->seed[cur]};
+    a.prevHitw = c->hitw[prev];
+    a.prevDist = c->dist[prev];
+    a.A = ResPlanes{c->ra[0], c->rb[0], c->rm[0]};
+    a.B = ResPlanes{c->ra[1], c->rb[1], c->rm[1]};
+    a.frame = c->frame;
+    a.emitLut = c->emitLut;
+    a.W = c->cfg.width, a.H = c->cfg.height;
+    a.point = c->cfg.light_mode == VR_LIGHTS_POINT ? 1u : 0u;
+    a.gfxexp = c->cfg.gfxexp ? 1u : 0u;
+    a.source = c->cfg.source_pdf;
+    a.axisOnlyFace = (c->cfg.quirks & VR_QUIRK_AXIS_ONLY_FACE) ? 1u : 0u;
+    a.doubleRadiance = (c->cfg.quirks & VR_QUIRK_DOUBLE_RADIANCE) ? 1u : 0u;
+    a.shadowSteps = (c->cfg.quirks & VR_QUIRK_SHADOW_STEP_CAP) ? 128 : 999999;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+void vr_default_config(vr_config* cfg, uint32_t width, uint32_t height)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->width = width, cfg->height = height;
+    cfg->light_mode = VR_LIGHTS_AREA;
+    cfg->gfxexp = 1;
+    cfg->quirks = VR_QUIRKS_REFERENCE;
+    cfg->source_pdf = VR_SOURCE_UNIFORM;
+    cfg->brick_capacity = VR_BRICKCOUNT;
+    for (int i = 0; i < 256; i++) cfg->emit_lut[i] = (float)i;
+}
+
+int vr_create(const vr_config* cfg, vr_ctx** out)
+{
+    if (!cfg || !out) return VR_ERR_ARG;
+    *out = nullptr;
+    if (cfg->width == 0 || cfg->height == 0 || cfg->width > 16384 || cfg->height > 16384) return VR_ERR_ARG;
+    if (cfg->brick_capacity == 0 || cfg->brick_capacity > (1u << 22)) return VR_ERR_ARG;
+    if (cfg->source_pdf > VR_SOURCE_POWER || cfg->light_mode > VR_LIGHTS_POINT) return VR_ERR_ARG;
+    if (cfg->emit_lut[0] != 0.0f) return VR_ERR_ARG;
+    vr_ctx* c = new (std::nothrow) vr_ctx();
+    if (!c) return VR_ERR_NOMEM;
+    c->cfg = *cfg;
+    const size_t n = (size_t)cfg->width * cfg->height;
+    auto ok = [&](hipError_t e) { return e == hipSuccess; };
+    bool good = ok(hipGetDevice(&c->device)) && ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int f = 0; good && f < kRing; f++)
+        for (int i = 0; good && i <= VR_NUM_STAGES; i++) good = ok(hipEventCreate(&c->ev[f][i]));
+    good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 32768)) &&
+           ok(dalloc(&c->bricks, (size_t)cfg->brick_capacity * VR_BRICKSIZE)) &&
+           ok(dalloc(&c->stageIds, VR_MAXCOMMITS)) && ok(dalloc(&c->stageData, (size_t)VR_MAXCOMMITS * VR_BRICKSIZE)) &&
+           ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n));
+    for (int i = 0; good && i < 2; i++)
+        good = ok(dalloc(&c->hitw[i], n)) && ok(dalloc(&c->dist[i], n)) && ok(dalloc(&c->seed[i], n)) &&
+               ok(dalloc(&c->ra[i], n)) && ok(dalloc(&c->rb[i], n)) && ok(dalloc(&c->rm[i], n));
+    if (good) {
+        good = ok(hipMemsetAsync(c->grid, 0, (size_t)VR_GRIDSIZE * 4, c->stream)) &&
+               ok(hipMemsetAsync(c->uber, 0, 32768, c->stream)) &&
+               ok(hipMemsetAsync(c->bricks, 0, (size_t)cfg->brick_capacity * VR_BRICKSIZE * 4, c->stream)) &&
+               ok(hipMemcpyAsync(c->emitLut, cfg->emit_lut, 256 * 4, hipMemcpyHostToDevice, c->stream)) &&
+               ok(hipMemsetAsync(c->frame, 0, n * 16, c->stream));
+        for (int i = 0; good && i < 2; i++)
+            good = ok(hipMemsetAsync(c->hitw[i], 0, n * 4, c->stream)) && ok(hipMemsetAsync(c->dist[i], 0, n * 4, c->stream)) &&
+                   ok(hipMemsetAsync(c->seed[i], 0, n * 4, c->stream)) && ok(hipMemsetAsync(c->ra[i], 0, n * 16, c->stream)) &&
+                   ok(hipMemsetAsync(c->rb[i], 0, n * 16, c->stream)) && ok(hipMemsetAsync(c->rm[i], 0, n * 4, c->stream));
+        good = good && ok(hipStreamSynchronize(c->stream));
+    }
+    if (!good) {
+        vr_destroy(c);
+        return VR_ERR_NOMEM;
+    }
+    *out = c;
+    return VR_OK;
+}
+
+int vr_destroy(vr_ctx* c)
+{
+    if (!c) return VR_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->grid), dfree(c->uber), dfree(c->bricks), dfree(c->stageIds), dfree(c->stageData);
+    dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf), dfree(c->emitLut), dfree(c->frame);
+    for (int i = 0; i < 2; i++) {
+        dfree(c->hitw[i]), dfree(c->dist[i]), dfree(c->seed[i]);
+        dfree(c->ra[i]), dfree(c->rb[i]), dfree(c->rm[i]);
+    }
+    for (int i = 0; i <= VR_NUM_STAGES; i++)
+        for (int f = 0; f < kRing; f++)
+            if (c->ev[f][i]) (void)hipEventDestroy(c->ev[f][i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VR_OK;
+}
+
+const char* vr_last_error(const vr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int vr_commit(vr_ctx* c, const uint32_t* grid, const uint32_t* ids, const uint32_t* payload, uint32_t n)
+{
+    if (!c) return VR_ERR_ARG;
+    if (!grid) return fail(c, VR_ERR_ARG, "vr_commit: grid is null");
+    if (n > 0 && (!ids || !payload)) return fail(c, VR_ERR_ARG, "vr_commit: brick ids/payload null");
+    const uint32_t cap = c->cfg.brick_capacity;
+    // every brick cell must name a brick inside the pool: the traversal
+    // indexes the pool with it
+    for (uint32_t i = 0; i < VR_GRIDSIZE; i++)
+        if ((grid[i] & 1u) && (grid[i] >> 1) >= cap)
+            return fail(c, VR_ERR_ARG, "vr_commit: grid cell " + std::to_string(i) + " names brick " +
+                                           std::to_string(grid[i] >> 1) + " beyond capacity " + std::to_string(cap));
+    for (uint32_t i = 0; i < n; i++)
+        if (ids[i] >= cap)
+            return fail(c, VR_ERR_ARG, "vr_commit: brick id " + std::to_string(ids[i]) + " beyond capacity");
+    HIPCHK(c, hipMemcpyAsync(c->grid, grid, (size_t)VR_GRIDSIZE * 4, hipMemcpyHostToDevice, c->stream));
+    for (uint32_t done = 0; done < n;) {
+        const uint32_t k = (n - done) < VR_MAXCOMMITS ? (n - done) : VR_MAXCOMMITS;
+        HIPCHK(c, hipMemcpyAsync(c->stageIds, ids + done, (size_t)k * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->stageData, payload + (size_t)done * VR_BRICKSIZE, (size_t)k * VR_BRICKSIZE * 4,
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_commit(c->stageIds, c->stageData, c->bricks, k, cap, c->stream));
+        done += k;
+    }
+    HIPCHK(c, launch_uber(c->grid, c->uber, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->committed = true;
+    return VR_OK;
+}
+
+int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
+{
+    if (!c) return VR_ERR_ARG;
+    if (n > 0 && !lights) return fail(c, VR_ERR_ARG, "vr_set_lights: null lights");
+    std::vector<float4> ha(n), hb(n);
+    for (uint32_t i = 0; i < n; i++) {
+        if (lights[i].position >= (1u << 30) || lights[i].size == 0 || lights[i].size > 1024)
+            return fail(c, VR_ERR_ARG, "vr_set_lights: light " + std::to_string(i) + " out of range");
+        light_record(c->cfg, lights[i], ha[i], hb[i]);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n > c->lightCap) {
+        dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf);
+        c->la = c->lb = nullptr, c->prob = c->pmf = nullptr, c->alias = nullptr;
+        const uint32_t cap = n * 2;
+        if (dalloc(&c->la, cap) != hipSuccess || dalloc(&c->lb, cap) != hipSuccess || dalloc(&c->prob, cap) != hipSuccess ||
+            dalloc(&c->alias, cap) != hipSuccess || dalloc(&c->pmf, cap) != hipSuccess) {
+            c->lightCap = 0;
+            return fail(c, VR_ERR_NOMEM, "vr_set_lights: device allocation failed");
+        }
+        c->lightCap = cap;
+    }
+    c->nLights = n;
+    if (n == 0) return VR_OK;
+    HIPCHK(c, hipMemcpyAsync(c->la, ha.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->lb, hb.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    if (c->cfg.source_pdf == VR_SOURCE_POWER) {
+        std::vector<float> prob, pmf;
+        std::vector<uint32_t> alias;
+        power_table(c->cfg, lights, n, prob, alias, pmf);
+        HIPCHK(c, hipMemcpyAsync(c->prob, prob.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->alias, alias.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->pmf, pmf.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
+}
+
+int vr_render_frame(vr_ctx* c, const vr_render_params* p)
+{
+    if (!c || !p) return VR_ERR_ARG;
+    if (!c->committed) return fail(c, VR_ERR_STATE, "vr_render_frame: no world committed");
+    if (p->skyDomeSampling) return fail(c, VR_ERR_UNSUPPORTED, "vr_render_frame: sky dome sampling is not supported");
+    if (p->numberOfLights != c->nLights)
+        return fail(c, VR_ERR_ARG, "vr_render_frame: numberOfLights " + std::to_string(p->numberOfLights) +
+                                       " differs from the uploaded light count " + std::to_string(c->nLights));
+    if (p->spatialRadius == 0 && p->spatial && p->spatialTaps)
+        return fail(c, VR_ERR_ARG, "vr_render_frame: spatialRadius must be >= 1");
+    if (!finite_camera(p)) return fail(c, VR_ERR_ARG, "vr_render_frame: non-finite camera parameters");
+    c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
+    const FrameArgs a = frame_args(c, p);
+    const int slot = (int)(c->frameNo % kRing);
+    {
+        const int rc = harvest(c, slot);
+        if (rc != VR_OK) return rc;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[slot][0], c->stream));
+    for (int s = 0; s < VR_NUM_STAGES; s++) {
+        HIPCHK(c, launch_stage(s, a, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[slot][s + 1], c->stream));
+    }
+    c->pending[slot] = c->profiling;
+    c->lastSlot = slot;
+    c->frameNo++;
+    c->framed = true;
+    return VR_OK;
+}
+
+int vr_sync(vr_ctx* c)
+{
+    if (!c) return VR_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
+}
+
+int vr_read_hits(vr_ctx* c, uint32_t* hitw, float* dist, uint32_t* seed)
+{
+    if (!c) return VR_ERR_ARG;
+    const size_t n = (size_t)c->cfg.width * c->cfg.height;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hitw) HIPCHK(c, hipMemcpy(hitw, c->hitw[c->cur], n * 4, hipMemcpyDeviceToHost));
+    if (dist) HIPCHK(c, hipMemcpy(dist, c->dist[c->cur], n * 4, hipMemcpyDeviceToHost));
+    if (seed) HIPCHK(c, hipMemcpy(seed, c->seed[c->cur], n * 4, hipMemcpyDeviceToHost));
+    return VR_OK;
+}
+
+int vr_read_reservoirs(vr_ctx* c, int which, float* a4, float* b4, uint32_t* m)
+{
+    if (!c || which < 0 || which > 1) return VR_ERR_ARG;
+    const size_t n = (size_t)c->cfg.width * c->cfg.height;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (a4) HIPCHK(c, hipMemcpy(a4, c->ra[which], n * 16, hipMemcpyDeviceToHost));
+    if (b4) HIPCHK(c, hipMemcpy(b4, c->rb[which], n * 16, hipMemcpyDeviceToHost));
+    if (m) HIPCHK(c, hipMemcpy(m, c->rm[which], n * 4, hipMemcpyDeviceToHost));
+    return VR_OK;
+}
+
+int vr_read_frame(vr_ctx* c, float* rgba)
+{
+    if (!c || !rgba) return VR_ERR_ARG;
+    const size_t n = (size_t)c->cfg.width * c->cfg.height;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(rgba, c->frame, n * 16, hipMemcpyDeviceToHost));
+    return VR_OK;
+}
+
+int vr_stage_times(vr_ctx* c, float ms[VR_NUM_STAGES])
+{
+    if (!c || !ms) return VR_ERR_ARG;
+    if (!c->framed) return fail(c, VR_ERR_STATE, "vr_stage_times: no frame rendered");
+    const int slot = c->lastSlot;
+    HIPCHK(c, hipEventSynchronize(c->ev[slot][VR_NUM_STAGES]));
+    for (int s = 0; s < VR_NUM_STAGES; s++)
+        HIPCHK(c, hipEventElapsedTime(&ms[s], c->ev[slot][s], c->ev[slot][s + 1]));
+    return VR_OK;
+}
+
+int vr_profile_begin(vr_ctx* c)
+{
+    if (!c) return VR_ERR_ARG;
+    c->profiling = false;
+    for (int f = 0; f < kRing; f++) c->pending[f] = false;
+    for (int s = 0; s < VR_NUM_STAGES; s++) c->profSum[s] = 0.0;
+    c->profFrames = 0;
+    c->profiling = true;
+    return VR_OK;
+}
+
+int vr_profile_read(vr_ctx* c, double sum_ms[VR_NUM_STAGES], uint32_t* frames)
+{
+    if (!c || !sum_ms || !frames) return VR_ERR_ARG;
+    for (int f = 0; f < kRing; f++) {
+        const int rc = harvest(c, f);
+        if (rc != VR_OK) return rc;
+    }
+    for (int s = 0; s < VR_NUM_STAGES; s++) sum_ms[s] = c->profSum[s];
+    *frames = c->profFrames;
+    return VR_OK;
+}
+
+int vr_trace_rays(vr_ctx* c, const float* O, const float* D, uint32_t n, uint32_t kind, int32_t steps,
+                  uint32_t* voxel, float* dist, uint32_t* side)
+{
+    if (!c) return VR_ERR_ARG;
+    if (n == 0) return VR_OK;
+    if (!O || !D || !voxel || !dist || !side || kind > VR_TRACE_OPAQUE || n > (1u << 26))
+        return fail(c, VR_ERR_ARG, "vr_trace_rays: bad arguments");
+    if (!c->committed) return fail(c, VR_ERR_STATE, "vr_trace_rays: no world committed");
+    for (size_t i = 0; i < (size_t)n * 3; i++)
+        if (!std::isfinite(O[i]) || !std::isfinite(D[i]))
+            return fail(c, VR_ERR_ARG, "vr_trace_rays: non-finite ray " + std::to_string(i / 3));
+    float *dO = nullptr, *dD = nullptr, *dd = nullptr;
+    uint32_t *dv = nullptr, *ds = nullptr;
+    int rc = VR_OK;
+    if (dalloc(&dO, (size_t)n * 3) != hipSuccess || dalloc(&dD, (size_t)n * 3) != hipSuccess ||
+        dalloc(&dd, n) != hipSuccess || dalloc(&dv, n) != hipSuccess || dalloc(&ds, n) != hipSuccess) {
+        rc = fail(c, VR_ERR_NOMEM, "vr_trace_rays: allocation failed");
+    } else {
+        hipError_t e = hipMemcpyAsync(dO, O, (size_t)n * 12, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dD, D, (size_t)n * 12, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = launch_trace_batch(WorldView{c->grid, c->bricks, c->uber}, dO, dD, n, kind, steps, dv, dd, ds, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(voxel, dv, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dist, dd, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(side, ds, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hipfail(c, e, "vr_trace_rays");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dfree(dO), dfree(dD), dfree(dd), dfree(dv), dfree(ds);
+    return rc;
+}
+
+int vr_ground_truth(vr_ctx* c, const vr_render_params* p, uint32_t spp, uint32_t frame0, double* accum)
+{
+    if (!c || !p || !accum) return VR_ERR_ARG;
+    if (!c->committed) return fail(c, VR_ERR_STATE, "vr_ground_truth: no world committed");
+    if (!finite_camera(p)) return fail(c, VR_ERR_ARG, "vr_ground_truth: non-finite camera parameters");
+    const size_t n = (size_t)c->cfg.width * c->cfg.height;
+    double* dacc = nullptr;
+    if (dalloc(&dacc, n * 3) != hipSuccess) return fail(c, VR_ERR_NOMEM, "vr_ground_truth: allocation failed");
+    int rc = VR_OK;
+    hipError_t e = hipMemcpyAsync(dacc, accum, n * 24, hipMemcpyHostToDevice, c->stream);
+    // chunks of at most 64 samples per launch; chunk k starts at R0 state
+    // xorshift32^(64k)(R0) and frame index frame0 + 64k
+    vr_render_params q = *p;
+    uint32_t done = 0;
+    while (e == hipSuccess && done < spp) {
+        const uint32_t k = (spp - done) < 64u ? (spp - done) : 64u;
+        FrameArgs a = frame_args(c, &q);
+        e = launch_ground_truth(a, k, frame0 + done, dacc, c->stream);
+        for (uint32_t i = 0; i < k; i++) {
+            uint32_t s = q.R0;
+            s ^= s << 13, s ^= s >> 17, s ^= s << 5;
+            q.R0 = s;
+        }
+        done += k;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(accum, dacc, n * 24, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hipfail(c, e, "vr_ground_truth");
+    (void)hipStreamSynchronize(c->stream);
+    dfree(dacc);
+    return rc;
+}
+
+}  // extern "C"
