@@ -55,7 +55,8 @@ class Alias(C.Structure):
 class State(C.Structure):
     _fields_ = [("W", C.c_uint32), ("H", C.c_uint32), ("hits", C.c_void_p * 2), ("res", C.c_void_p * 2),
                 ("frame", C.c_void_p), ("hitCur", C.c_int), ("hitPrev", C.c_int), ("resOut", C.c_int),
-                ("resIn", C.c_int), ("initialOut", C.c_int), ("spatialOut", C.c_int), ("shadeIn", C.c_int)]
+                ("resIn", C.c_int), ("initialOut", C.c_int), ("spatialOut", C.c_int), ("shadeIn", C.c_int),
+                ("initial", C.c_void_p)]
 
 
 _lib = None
@@ -309,7 +310,7 @@ class Frames:
 
     def initial(self) -> np.ndarray:
         """Reservoirs after candidates + temporal reuse of the last frame."""
-        return self._arr(self.st.res[self.st.initialOut], RES_DTYPE)
+        return self._arr(self.st.initial, RES_DTYPE)
 
     def final(self) -> np.ndarray:
         """Reservoirs after spatial reuse and shading's discard."""

@@ -852,6 +852,7 @@ or_state* or_state_create(uint32_t W, uint32_t H)
         s->res[i] = (or_reservoir*)calloc(n, sizeof(or_reservoir));
     }
     s->frame = (float*)calloc(n * 4, sizeof(float));
+    s->initial = (or_reservoir*)calloc(n, sizeof(or_reservoir));
     s->hitCur = 0, s->hitPrev = 1;
     s->resOut = 0, s->resIn = 1;
     return s;
@@ -862,6 +863,7 @@ void or_state_destroy(or_state* s)
     if (!s) return;
     for (int i = 0; i < 2; i++) free(s->hits[i]), free(s->res[i]);
     free(s->frame);
+    free(s->initial);
     free(s);
 }
 
@@ -890,6 +892,7 @@ void or_render_frame(or_state* s, const or_config* c, const or_params* p,
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (int y = 0; y < H; y += rows)
         or_stage_initial(c, p, w, lights, src, prev, cur, s->res[iOut], s->res[iIn], y, y + rows < H ? y + rows : H);
+    memcpy(s->initial, s->res[iOut], (size_t)s->W * s->H * sizeof(or_reservoir));
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (int y = 0; y < H; y += rows) or_stage_spatial(c, p, lights, cur, s->res[sOut], s->res[sIn], y, y + rows < H ? y + rows : H);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)

@@ -179,6 +179,9 @@ typedef struct or_state {
     int hitCur, hitPrev;    /* world.cpp:2221-2223 statics */
     int resOut, resIn;      /* world.cpp:2209-2219 statics */
     int initialOut, spatialOut, shadeIn;  /* buffer indices used by the last frame */
+    or_reservoir* initial;  /* copy of the last frame's candidates+temporal output
+                               (the rotation overwrites that buffer with the
+                               shading stage's copy, ris.cl:412) */
 } or_state;
 
 or_state* or_state_create(uint32_t W, uint32_t H);

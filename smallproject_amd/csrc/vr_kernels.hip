@@ -69,6 +69,14 @@ __device__ __forceinline__ bool same_face(uint32_t a, uint32_t b, bool axis_only
 
 // RandomPointOnVoxel (cl/tools.cl:499-577): 12 draws for the six faces,
 // one for the face CDF.
+//
+// Face i has the axis normal N = s*e_a, so dot(N, -L) = -s*L_a exactly (the
+// two zero products only add signed zeros).  When the shading point is not
+// strictly in front of the face plane (s*R_a >= 0) that value is <= 0, or NaN
+// for R = 0, and max(0, .) makes the face's solid angle zero: the square root
+// and both divisions are skipped for those faces with identical results (a
+// zero's sign never reaches an output: it only enters sums and the 1/pdf
+// that is zeroed when infinite).
 __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t& seed, f3& pos, float& invPdf,
                                                uint32_t& face)
 {
@@ -76,25 +84,35 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
     f3 pts[6];
     float total = 0.0f;
     const float hs = 0.5f * size;
+    const float area = size * size;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const f3 N = face_normal((uint32_t)i);
-        const int ti = i < 3 ? (i + 1) % 3 : 3 + (i - 2) % 3;   // 1,2,0,4,5,3
-        const int bi = i < 3 ? (i + 2) % 3 : 3 + (i - 1) % 3;   // 2,0,1,5,3,4
-        const f3 T = face_normal((uint32_t)ti), Bn = face_normal((uint32_t)bi);
+        // normals +x,+y,+z,-x,-y,-z; tangent axis (a+1)%3, bitangent (a+2)%3
+        // with the face's sign (tools.cl:502-518)
+        const int a = i % 3, ta = (a + 1) % 3, ba = (a + 2) % 3;
+        const float s = i < 3 ? 1.0f : -1.0f;
         const float r0 = rnd(seed), r1 = rnd(seed);
         const float s0 = r0 * size, s1 = r1 * size;
-        const f3 pt = f3{c.x + hs * N.x - hs * T.x - hs * Bn.x + s0 * T.x + s1 * Bn.x,
-                         c.y + hs * N.y - hs * T.y - hs * Bn.y + s0 * T.y + s1 * Bn.y,
-                         c.z + hs * N.z - hs * T.z - hs * Bn.z + s0 * T.z + s1 * Bn.z};
-        const f3 R = sub(pt, sp);
-        const f3 L = normalize(R);
-        const float d2 = fmaxf(1e-5f, dot(R, R));
-        const float ndl = dot(N, neg(L));
-        const float sa = ndl * (size * size) / d2;
+        // centre + hs*N - hs*T - hs*B + r0*size*T + r1*size*B per axis; the
+        // zero components of N, T, B leave the other terms unchanged
+        float p[3] = {c.x, c.y, c.z};
+        p[a] = p[a] + hs * s;
+        p[ta] = (p[ta] - hs * s) + s0 * s;
+        p[ba] = (p[ba] - hs * s) + s1 * s;
+        const f3 pt = f3{p[0], p[1], p[2]};
         pts[i] = pt;
-        solid[i] = fmaxf(0.0f, sa);
-        total += fmaxf(0.0f, sa);
+        const f3 R = sub(pt, sp);
+        const float Ra = a == 0 ? R.x : (a == 1 ? R.y : R.z);
+        float sol = 0.0f;
+        if (s * Ra < 0.0f) {
+            const float d = dot(R, R);
+            const float inv = 1.0f / sqrtf(d);
+            const float ndl = -s * (Ra * inv);
+            const float d2 = fmaxf(1e-5f, d);
+            sol = fmaxf(0.0f, ndl * area / d2);
+        }
+        solid[i] = sol;
+        total += sol;
     }
     const float r2 = rnd(seed);
     uint32_t index = 0;
