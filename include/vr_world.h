@@ -80,6 +80,17 @@ int vrw_commit(vrw_world* w);
 int vrw_force_sync(vrw_world* w);
 int vrw_stage_times(vrw_world* w, float ms[VR_NUM_STAGES]);
 
+/* ---- scene files: the reference's gzip .vx format ---- */
+/* LoadSprite + StampSpriteTo (template/world.cpp:1005-1038, 1547-1563): stamp
+ * frame 0 of a .vx file at (x,y,z).  16-bit MRGB4444 payloads become
+ * 0x00MMARGB with A=F and the emission nibble as emission byte; 8-bit RGB332
+ * payloads are expanded to 4-bit channels.  stats (optional, 4 u32): entries,
+ * non-zero entries, payload size, frames. */
+int vrw_load_vx(vrw_world* w, const char* path, int x, int y, int z, uint32_t* stats);
+/* CreateSprite over a box + SaveSprite (world.cpp:1307-1325, 1347-1386),
+ * 32-bit payload */
+int vrw_save_vx(const vrw_world* w, const char* path, int x, int y, int z, int sx, int sy, int sz);
+
 /* ---- seeded scene generators of the benchmark configurations ---- */
 /* value-noise heightfield over [0,size)^2: h = scale*(40 + 24*n(x/scale, z/scale))
  * with n a 4-octave value noise normalised to [-1,1] */

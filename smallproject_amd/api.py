@@ -44,17 +44,21 @@ def focal_for_fov(fov_deg: float) -> float:
 class Renderer:
     """A vr_ctx: device world + per-pixel state + frame pipeline."""
 
-    def __init__(self, cfg: VrConfig):
+    def __init__(self, cfg: VrConfig, _borrowed_handle=None):
         self.cfg = cfg
+        self.n = cfg.width * cfg.height
+        if _borrowed_handle is not None:
+            # a context owned by a World: never destroyed from here
+            self.h, self._owned = _borrowed_handle, False
+            return
         h = C.c_void_p()
         check(lib().vr_create(C.byref(cfg), C.byref(h)), None, "vr_create")
-        self.h = h.value
-        self.n = cfg.width * cfg.height
+        self.h, self._owned = h.value, True
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owned", False):
             lib().vr_destroy(self.h)
-            self.h = None
+        self.h = None
 
     __del__ = close
 
@@ -143,6 +147,8 @@ class World:
 
     def close(self):
         if getattr(self, "h", None):
+            if getattr(self, "_renderer", None) is not None:
+                self._renderer.h = None
             lib().vrw_destroy(self.h)
             self.h = None
 
@@ -275,11 +281,17 @@ class World:
             ctx = lib().vrw_context(self.h)
             if not ctx:
                 raise _lib.VrError(_lib.VR_ERR_STATE, "no device context: " + (lib().vrw_last_error(self.h) or b"").decode())
-            r = Renderer.__new__(Renderer)
-            r.cfg, r.h, r.n = self.cfg, ctx, self.cfg.width * self.cfg.height
-            r.close = lambda: None  # owned by the World
-            self._renderer = r
+            self._renderer = Renderer(self.cfg, _borrowed_handle=ctx)
         return self._renderer
+
+    # scene files
+    def load_vx(self, path, x: int = 0, y: int = 0, z: int = 0) -> dict:
+        stats = np.zeros(4, np.uint32)
+        self._chk(lib().vrw_load_vx(self.h, str(path).encode(), x, y, z, ptr(stats)), f"load_vx {path}")
+        return {"entries": int(stats[0]), "nonzero": int(stats[1]), "payload": int(stats[2]), "frames": int(stats[3])}
+
+    def save_vx(self, path, x: int, y: int, z: int, sx: int, sy: int, sz: int):
+        self._chk(lib().vrw_save_vx(self.h, str(path).encode(), x, y, z, sx, sy, sz), f"save_vx {path}")
 
     # scenes
     def scene_terrain(self, size: int, scale: int, seed: int):
