@@ -91,6 +91,15 @@ int vrw_load_vx(vrw_world* w, const char* path, int x, int y, int z, uint32_t* s
  * 32-bit payload */
 int vrw_save_vx(const vrw_world* w, const char* path, int x, int y, int z, int sx, int sy, int sz);
 
+/* ---- image metrics and raster dumps (mse.cpp:33-65, mygame.cpp:518-549) ---- */
+/* out[0] = MSE, out[1] = mean absolute error over the rgb channels of two
+ * float4 rasters: per-pixel differences in binary32, sums in binary64 */
+int vrw_image_error(const float* gt, const float* img, uint32_t width, uint32_t height, double out[2]);
+/* .dat raster: u32 width, u32 height, float4[width*height] */
+int vrw_write_dat(const char* path, uint32_t width, uint32_t height, const float* rgba);
+/* reads the size; pixels too when rgba is non-null and holds capacity_px */
+int vrw_read_dat(const char* path, uint32_t* width, uint32_t* height, float* rgba, size_t capacity_px);
+
 /* ---- seeded scene generators of the benchmark configurations ---- */
 /* value-noise heightfield over [0,size)^2: h = scale*(40 + 24*n(x/scale, z/scale))
  * with n a 4-octave value noise normalised to [-1,1] */

@@ -113,6 +113,10 @@ def lib() -> C.CDLL:
             "or_world_brick_high_water": (u32, [P]),
             "or_world_dirty": (u32, [P, P, u32]),
             "or_build_uber": (None, [P, P]),
+            "or_mlp_num_params": (u32, []),
+            "or_mlp_forward": (None, [P, P, u32, P, P]),
+            "or_mlp_loss_grad": (C.c_double, [P, P, P, P, u32, P]),
+            "or_adam_step": (None, [P, P, P, P, u32, C.c_double, C.c_double, C.c_double, C.c_double, u32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -327,3 +331,38 @@ def ground_truth(cfg: Config, params: Params, view: View, spp: int, frame0: int 
     y1 = cfg.height if y1 is None else y1
     lib().or_ground_truth(C.byref(cfg), C.byref(params), view.ref, spp, frame0, ptr(acc), y0, y1)
     return acc.reshape(-1, 3)
+
+
+# ---- learned proposal network (oracle/vr_mlp.c) ---------------------------
+MLP_SIZES = (9, 64, 64, 256)
+
+
+def mlp_num_params() -> int:
+    return lib().or_mlp_num_params()
+
+
+def mlp_forward(P: np.ndarray, x: np.ndarray):
+    """(logits, probabilities) for inputs x[B, 9] in binary64."""
+    P = np.ascontiguousarray(P, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 9)
+    B = x.shape[0]
+    z = np.zeros((B, 256), np.float64)
+    q = np.zeros((B, 256), np.float64)
+    lib().or_mlp_forward(ptr(P), ptr(x), B, ptr(z), ptr(q))
+    return z, q
+
+
+def mlp_loss_grad(P: np.ndarray, x: np.ndarray, t: np.ndarray, w: np.ndarray | None = None):
+    P = np.ascontiguousarray(P, np.float64)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 9)
+    t = np.ascontiguousarray(t, np.uint32)
+    wv = None if w is None else np.ascontiguousarray(w, np.float64)
+    g = np.zeros_like(P)
+    loss = lib().or_mlp_loss_grad(ptr(P), ptr(x), ptr(t), ptr(wv) if wv is not None else None, x.shape[0], ptr(g))
+    return loss, g
+
+
+def adam_step(P, g, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    for a in (P, g, m, v):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    lib().or_adam_step(ptr(P), ptr(g), ptr(m), ptr(v), t, lr, b1, b2, eps, P.size)

@@ -22,7 +22,7 @@ FLAGS = ["-O3", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-
 
 def build() -> Path:
     tmp = HERE / "liboracle.so.tmp"
-    cmd = ["gcc", *FLAGS, str(HERE / "vr_oracle.c"), "-o", str(tmp), "-lm"]
+    cmd = ["gcc", *FLAGS, str(HERE / "vr_oracle.c"), str(HERE / "vr_mlp.c"), "-o", str(tmp), "-lm"]
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{res.stdout}")

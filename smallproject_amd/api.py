@@ -308,6 +308,31 @@ class World:
         self._chk(lib().vrw_scene_lights_surface(self.h, n, size, seed, mmin, mmax), "scene_lights_surface")
 
 
+def image_error(gt: np.ndarray, img: np.ndarray, width: int, height: int) -> tuple[float, float]:
+    """(MSE, mean absolute error) of two float4 rasters (mse.cpp:33-65)."""
+    a = np.ascontiguousarray(gt, np.float32).reshape(-1, 4)
+    b = np.ascontiguousarray(img, np.float32).reshape(-1, 4)
+    assert a.shape[0] == b.shape[0] == width * height
+    out = np.zeros(2, np.float64)
+    check(lib().vrw_image_error(ptr(a), ptr(b), width, height, ptr(out)), None, "image_error")
+    return float(out[0]), float(out[1])
+
+
+def write_dat(path, width: int, height: int, rgba: np.ndarray):
+    a = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
+    assert a.shape[0] == width * height
+    check(lib().vrw_write_dat(str(path).encode(), width, height, ptr(a)), None, f"write_dat {path}")
+
+
+def read_dat(path) -> np.ndarray:
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().vrw_read_dat(str(path).encode(), C.byref(w), C.byref(h), None, 0), None, f"read_dat {path}")
+    out = np.zeros((h.value * w.value, 4), np.float32)
+    check(lib().vrw_read_dat(str(path).encode(), C.byref(w), C.byref(h), ptr(out), out.shape[0]), None,
+          f"read_dat {path}")
+    return out.reshape(h.value, w.value, 4)
+
+
 def terrain_height(x: float, z: float, scale: int, seed: int) -> float:
     return lib().vrw_terrain_height(C.c_float(x), C.c_float(z), scale, seed)
 
