@@ -68,6 +68,24 @@ def cpu_baseline(sc, nthreads: int) -> dict:
                       f"{sc.name} at {sc.width}x{sc.height}; frame 1 (with temporal reuse) timed"}
 
 
+def calibrate(launches: int = 10) -> int:
+    """A plain copy of known size for scaling the HBM byte counters: the
+    commit kernel (k_commit) copying 8192 staged bricks (16 MiB) into the
+    brick pool, `launches` times.  Prints the bytes each launch moves."""
+    from smallproject_amd import Renderer, default_config
+    r = Renderer(default_config(64, 64))
+    n = 8192
+    grid = np.zeros(128 ** 3, np.uint32)
+    ids = np.arange(n, dtype=np.uint32) * 3
+    payload = np.arange(n * 512, dtype=np.uint32)
+    for _ in range(launches):
+        r.commit(grid, ids, payload)
+    r.sync()
+    print(json.dumps({"calibration_kernel": "k_commit", "launches": launches,
+                      "read_bytes": n * 2048 + n * 4, "write_bytes": n * 2048}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,11 +94,15 @@ def main() -> int:
     ap.add_argument("--config", default="c2_area_1024")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run only the known-size copy used to scale HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the render path runs on a single MI355X)", file=sys.stderr)
         return 2
+    if args.calibrate:
+        return calibrate()
 
     from smallproject_amd import configs
     sc = configs.CONFIGS[args.config]

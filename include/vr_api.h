@@ -49,6 +49,7 @@ extern "C" {
 /* candidate source pdf */
 #define VR_SOURCE_UNIFORM 0u  /* cl/tools.cl:593-598: p = 1/N (reference) */
 #define VR_SOURCE_POWER   1u  /* alias table over lightContribution (cl/tools.cl:579-591) */
+#define VR_SOURCE_LEARNED 2u  /* MLP proposal over light clusters (no reference counterpart) */
 
 /* reference quirks reproduced when set (VR_QUIRKS_REFERENCE = all three) */
 #define VR_QUIRK_DOUBLE_RADIANCE 1u /* cl/ris.cl:254-255                      */
@@ -163,6 +164,34 @@ int vr_profile_read(vr_ctx* ctx, double sum_ms[VR_NUM_STAGES], uint32_t* frames)
  * Outputs voxel, dist, side per ray (voxel 0 = miss, dist/side then 0). */
 int vr_trace_rays(vr_ctx* ctx, const float* origins, const float* dirs, uint32_t n,
                   uint32_t kind, int32_t steps, uint32_t* voxel, float* dist, uint32_t* side);
+
+/* ---- learned proposal (VR_SOURCE_LEARNED; no counterpart in the reference) ----
+ * A 9-64-64-256 MLP maps a shading point (position / 1024, one-hot voxel
+ * face) to a softmax over light clusters; a candidate is a cluster drawn
+ * from it, then a light drawn uniformly inside the cluster, with source pdf
+ * q(cluster)/|cluster| in the RIS weight p-hat/p.  The 32 candidates of a
+ * pixel are stratified over the cluster distribution.  BF16 MFMA with FP32
+ * accumulation for inference and training; FP32 master weights and Adam. */
+#define VR_MLP_PARAMS 21440u  /* W1[64][9] b1[64] W2[64][64] b2[64] W3[256][64] b3[256] */
+
+/* cluster of every uploaded light (0..nclusters-1, nclusters <= 256); must
+ * be called again after vr_set_lights */
+int vr_set_light_clusters(vr_ctx* ctx, const uint32_t* cluster_of_light, uint32_t nlights, uint32_t nclusters);
+/* FP32 master parameters in the flat layout above; setting them resets Adam */
+int vr_mlp_set_params(vr_ctx* ctx, const float* params, uint32_t n);
+int vr_mlp_get_params(vr_ctx* ctx, float* params, uint32_t n);
+/* cluster probabilities q[B][256] for inputs x[B][9] */
+int vr_mlp_probs(vr_ctx* ctx, const float* x, uint32_t B, float* q);
+/* gradient (VR_MLP_PARAMS floats) of (1/B) sum_b w_b * -log q(t_b | x_b) and the loss */
+int vr_mlp_grad(vr_ctx* ctx, const float* x, const uint32_t* t, const float* w, uint32_t B, float* grad,
+                double* loss);
+/* online training: gather nsamples (x, cluster of the final reservoir's
+ * light, weight 1 if the reservoir holds a visible sample) from the last
+ * rendered frame, then `steps` Adam steps (beta 0.9/0.999) at rate lr;
+ * *loss (optional) receives the last step's loss */
+int vr_mlp_train(vr_ctx* ctx, uint32_t nsamples, float lr, uint32_t steps, double* loss);
+/* the last frame's candidate slots: idx[32][W*H] light indices and pdf[32][W*H] */
+int vr_read_candidates(vr_ctx* ctx, uint32_t* idx, float* pdf);
 
 /* FP64 converged reference image: render_di_gfxexp (cl/kernels.cl:177-241)
  * accumulated over spp samples in binary64 (the reference's ACCUMULATOR=1

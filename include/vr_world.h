@@ -114,10 +114,22 @@ int vrw_scene_lights_above(vrw_world* w, uint32_t n, uint32_t size, uint32_t sca
                            uint32_t seed, uint32_t hmin, uint32_t hmax, uint32_t mmin, uint32_t mmax);
 /* n emissive voxels replacing the top voxel of uniformly drawn columns */
 int vrw_scene_lights_surface(vrw_world* w, uint32_t n, uint32_t size, uint32_t seed, uint32_t mmin, uint32_t mmax);
+/* flat floor (height floor_h, a multiple of 8) over [0,size)^2 plus n boxes
+ * with edges uniform in [emin, emax] standing on it */
+int vrw_scene_floor_boxes(vrw_world* w, uint32_t n, uint32_t emin, uint32_t emax, uint32_t size, uint32_t floor_h,
+                          uint32_t seed);
+/* n emissive voxels in nclusters Gaussian clusters (std. dev. sigma) with
+ * centres uniform over [0,size)^2 x [ymin,ymax), in empty voxels only */
+int vrw_scene_light_clusters(vrw_world* w, uint32_t n, uint32_t nclusters, float sigma, uint32_t size, uint32_t ymin,
+                             uint32_t ymax, uint32_t seed, uint32_t mmin, uint32_t mmax);
+/* k-means (k-means++ seeding, iters Lloyd steps) of the light list's
+ * positions; out[i] = cluster of light i (for vr_set_light_clusters) */
+int vrw_cluster_lights(const vrw_world* w, uint32_t k, uint32_t iters, uint32_t seed, uint32_t* out);
 /* emission LUT: lut[0] = 0, lut[m] = lo * (hi/lo)^((m-1)/254) for m >= 1 */
 void vrw_log_lut(float lo, float hi, float lut[256]);
 /* camera path: kind 0 = straight line at 2 voxels/frame toward the centre,
- * kind 1 = Catmull-Rom spline through seeded control points above the terrain */
+ * kind 1 = Catmull-Rom spline through seeded control points above the terrain
+ * (above y = 0 when scale is 0, for flat-floor scenes) */
 int vrw_camera_path(uint32_t kind, uint32_t seed, uint32_t frame, uint32_t nframes, uint32_t size, uint32_t scale,
                     uint32_t terrain_seed, float pos[3], float target[3]);
 

@@ -16,7 +16,9 @@ LIBPATH = PKG / "lib" / "libvoxelrestir.so"
 
 VR_OK, VR_ERR_ARG, VR_ERR_HIP, VR_ERR_STATE, VR_ERR_NOMEM, VR_ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 LIGHTS_AREA, LIGHTS_POINT = 0, 1
-SOURCE_UNIFORM, SOURCE_POWER = 0, 1
+SOURCE_UNIFORM, SOURCE_POWER, SOURCE_LEARNED = 0, 1, 2
+MLP_PARAMS = 21440
+CANDIDATES = 32
 QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
 QUIRKS_REFERENCE = 7
 TRACE_ALL, TRACE_OPAQUE = 0, 1
@@ -72,6 +74,13 @@ SIGNATURES = {
     "vr_profile_read": (C.c_int, [_P, _P, _P]),
     "vr_trace_rays": (C.c_int, [_P, _P, _P, _u32, _u32, _i32, _P, _P, _P]),
     "vr_ground_truth": (C.c_int, [_P, _P, _u32, _u32, _P]),
+    "vr_set_light_clusters": (C.c_int, [_P, _P, _u32, _u32]),
+    "vr_mlp_set_params": (C.c_int, [_P, _P, _u32]),
+    "vr_mlp_get_params": (C.c_int, [_P, _P, _u32]),
+    "vr_mlp_probs": (C.c_int, [_P, _P, _u32, _P]),
+    "vr_mlp_grad": (C.c_int, [_P, _P, _P, _P, _u32, _P, _P]),
+    "vr_mlp_train": (C.c_int, [_P, _u32, _f32, _u32, _P]),
+    "vr_read_candidates": (C.c_int, [_P, _P, _P]),
     # vr_world.h
     "vrw_create": (C.c_int, [_P, _P]),
     "vrw_destroy": (C.c_int, [_P]),
@@ -118,6 +127,9 @@ SIGNATURES = {
     "vrw_scene_lights_above": (C.c_int, [_P, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32]),
     "vrw_scene_lights_surface": (C.c_int, [_P, _u32, _u32, _u32, _u32, _u32]),
     "vrw_log_lut": (None, [_f32, _f32, _P]),
+    "vrw_scene_floor_boxes": (C.c_int, [_P, _u32, _u32, _u32, _u32, _u32, _u32]),
+    "vrw_scene_light_clusters": (C.c_int, [_P, _u32, _u32, _f32, _u32, _u32, _u32, _u32, _u32, _u32]),
+    "vrw_cluster_lights": (C.c_int, [_P, _u32, _u32, _u32, _P]),
     "vrw_camera_path": (C.c_int, [_u32, _u32, _u32, _u32, _u32, _u32, _u32, _P, _P]),
 }
 

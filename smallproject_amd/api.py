@@ -119,6 +119,48 @@ class Renderer:
         self._chk(lib().vr_profile_read(self.h, ptr(s), C.byref(n)), "vr_profile_read")
         return s, n.value
 
+    # learned proposal
+    def set_light_clusters(self, cluster_of_light: np.ndarray, nclusters: int):
+        c = np.ascontiguousarray(cluster_of_light, np.uint32)
+        self._chk(lib().vr_set_light_clusters(self.h, ptr(c) if len(c) else None, len(c), nclusters),
+                  "vr_set_light_clusters")
+
+    def mlp_set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == _lib.MLP_PARAMS
+        self._chk(lib().vr_mlp_set_params(self.h, ptr(p), p.size), "vr_mlp_set_params")
+
+    def mlp_get_params(self) -> np.ndarray:
+        p = np.zeros(_lib.MLP_PARAMS, np.float32)
+        self._chk(lib().vr_mlp_get_params(self.h, ptr(p), p.size), "vr_mlp_get_params")
+        return p
+
+    def mlp_probs(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 9)
+        q = np.zeros((x.shape[0], 256), np.float32)
+        self._chk(lib().vr_mlp_probs(self.h, ptr(x), x.shape[0], ptr(q)), "vr_mlp_probs")
+        return q
+
+    def mlp_grad(self, x: np.ndarray, t: np.ndarray, w: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(t, np.uint32)
+        w = np.ascontiguousarray(w, np.float32)
+        g = np.zeros(_lib.MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        self._chk(lib().vr_mlp_grad(self.h, ptr(x), ptr(t), ptr(w), x.shape[0], ptr(g), C.byref(loss)), "vr_mlp_grad")
+        return loss.value, g
+
+    def mlp_train(self, nsamples: int = 1 << 16, lr: float = 1e-3, steps: int = 1) -> float:
+        loss = C.c_double()
+        self._chk(lib().vr_mlp_train(self.h, nsamples, C.c_float(lr), steps, C.byref(loss)), "vr_mlp_train")
+        return loss.value
+
+    def candidates(self):
+        idx = np.zeros((_lib.CANDIDATES, self.n), np.uint32)
+        pdf = np.zeros((_lib.CANDIDATES, self.n), np.float32)
+        self._chk(lib().vr_read_candidates(self.h, ptr(idx), ptr(pdf)), "vr_read_candidates")
+        return idx, pdf
+
     def trace(self, O: np.ndarray, D: np.ndarray, kind: int = _lib.TRACE_ALL, steps: int = 999999):
         O = np.ascontiguousarray(O, dtype=np.float32).reshape(-1, 3)
         D = np.ascontiguousarray(D, dtype=np.float32).reshape(-1, 3)
@@ -306,6 +348,19 @@ class World:
 
     def scene_lights_surface(self, n, size, seed, mmin, mmax):
         self._chk(lib().vrw_scene_lights_surface(self.h, n, size, seed, mmin, mmax), "scene_lights_surface")
+
+    def scene_floor_boxes(self, n, emin, emax, size, floor_h, seed):
+        self._chk(lib().vrw_scene_floor_boxes(self.h, n, emin, emax, size, floor_h, seed), "scene_floor_boxes")
+
+    def scene_light_clusters(self, n, nclusters, sigma, size, ymin, ymax, seed, mmin, mmax):
+        self._chk(lib().vrw_scene_light_clusters(self.h, n, nclusters, sigma, size, ymin, ymax, seed, mmin, mmax),
+                  "scene_light_clusters")
+
+    def cluster_lights(self, k: int, iters: int = 8, seed: int = 1) -> np.ndarray:
+        """Cluster index of every light of the light list (k-means)."""
+        out = np.zeros(max(1, lib().vrw_num_lights(self.h)), np.uint32)
+        self._chk(lib().vrw_cluster_lights(self.h, k, iters, seed, ptr(out)), "cluster_lights")
+        return out[:lib().vrw_num_lights(self.h)]
 
 
 def image_error(gt: np.ndarray, img: np.ndarray, width: int, height: int) -> tuple[float, float]:

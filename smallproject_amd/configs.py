@@ -9,6 +9,11 @@ and render settings.  `build_world` constructs the host World (CPU only);
   c2_area_1024   plain Voxel ReSTIR, area lights (per-face solid-angle
                  sampling), 1024^3 heightfield x8 + 2000 boxes, 4096 emissive
                  surface voxels, 1920x1080, 60-frame spline path
+  c3_clusters_1024  many-light scene: 30,000 boxes (edges 2-24) on a flat
+                 floor, 65,536 emissive voxels in 256 Gaussian clusters
+                 (sigma 16), emission log-uniform in [1e-2, 1e2]; area
+                 lights, 1920x1080, 60-frame spline path; the source pdf is
+                 `source` (power-proportional or the learned proposal)
 """
 from __future__ import annotations
 
@@ -34,7 +39,7 @@ class SceneConfig:
     box_seed: int = 0
     lights: int = 0
     light_seed: int = 0
-    light_mode: str = "above"     # "above" (hmin..hmax over terrain) or "surface"
+    light_mode: str = "above"     # "above" (hmin..hmax over terrain), "surface" or "clusters"
     light_height: tuple = (2, 20)
     emit_range: tuple = (1, 10)   # emit byte range
     lut: tuple | None = None      # (lo, hi) log LUT or None for EmitStrength = byte
@@ -44,6 +49,10 @@ class SceneConfig:
     taps: int = 5
     radius: int = 30
     temporal_clamp: int = 20
+    floor: int = 0                # > 0: flat floor of this height instead of the heightfield
+    clusters: int = 0             # Gaussian light clusters ("clusters" mode) and proposal clusters
+    cluster_sigma: float = 16.0
+    cluster_y: tuple = (16, 128)
     quirks: int = _lib.QUIRKS_REFERENCE
     source: int = _lib.SOURCE_UNIFORM
     extra: dict = field(default_factory=dict)
@@ -67,6 +76,11 @@ CONFIGS = {
         name="c2_area_1024", width=1920, height=1080, point=False, fov=60.0, frames=60, world_size=1024, scale=8,
         terrain_seed=1, boxes=2000, box_edge=(4, 32), box_seed=4, lights=4096, light_seed=5,
         light_mode="surface", emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=6),
+    "c3_clusters_1024": SceneConfig(
+        name="c3_clusters_1024", width=1920, height=1080, point=False, fov=60.0, frames=60, world_size=1024,
+        scale=0, terrain_seed=0, floor=8, boxes=30000, box_edge=(2, 24), box_seed=7, lights=65536, light_seed=11,
+        light_mode="clusters", clusters=256, cluster_sigma=16.0, cluster_y=(16, 128), emit_range=(1, 255),
+        lut=(1e-2, 1e2), camera_kind=1, camera_seed=12, source=_lib.SOURCE_POWER),
 }
 
 
@@ -75,12 +89,19 @@ def build_world(sc: SceneConfig | str, width: int | None = None, height: int | N
     if isinstance(sc, str):
         sc = CONFIGS[sc]
     w = World(sc.config(width, height))
-    w.scene_terrain(sc.world_size, sc.scale, sc.terrain_seed)
-    if sc.boxes:
-        w.scene_boxes(sc.boxes, sc.box_edge[0], sc.box_edge[1], sc.world_size, sc.scale, sc.terrain_seed, sc.box_seed)
+    if sc.floor:
+        w.scene_floor_boxes(sc.boxes, sc.box_edge[0], sc.box_edge[1], sc.world_size, sc.floor, sc.box_seed)
+    else:
+        w.scene_terrain(sc.world_size, sc.scale, sc.terrain_seed)
+        if sc.boxes:
+            w.scene_boxes(sc.boxes, sc.box_edge[0], sc.box_edge[1], sc.world_size, sc.scale, sc.terrain_seed,
+                          sc.box_seed)
     w.optimize()
     if sc.lights:
-        if sc.light_mode == "above":
+        if sc.light_mode == "clusters":
+            w.scene_light_clusters(sc.lights, sc.clusters, sc.cluster_sigma, sc.world_size, sc.cluster_y[0],
+                                   sc.cluster_y[1], sc.light_seed, sc.emit_range[0], sc.emit_range[1])
+        elif sc.light_mode == "above":
             w.scene_lights_above(sc.lights, sc.world_size, sc.scale, sc.terrain_seed, sc.light_seed,
                                  sc.light_height[0], sc.light_height[1], sc.emit_range[0], sc.emit_range[1])
         else:

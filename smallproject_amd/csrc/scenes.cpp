@@ -170,6 +170,122 @@ int vrw_scene_lights_surface(vrw_world* wp, uint32_t n, uint32_t size, uint32_t 
     return placed == n ? VR_OK : VR_ERR_ARG;
 }
 
+int vrw_scene_floor_boxes(vrw_world* wp, uint32_t n, uint32_t emin, uint32_t emax, uint32_t size, uint32_t floor_h,
+                          uint32_t seed)
+{
+    // a flat floor of height floor_h over [0,size)^2 (solid cells) and n
+    // boxes with edges uniform in [emin, emax] standing on it
+    if (!wp || emin == 0 || emax < emin || size <= emax || size > 1024 || floor_h == 0 || floor_h % 8 != 0)
+        return VR_ERR_ARG;
+    vrhost::World& w = *reinterpret_cast<vrhost::World*>(wp);
+    const uint32_t floorc = rgb332_to_4444(0x6du);
+    for (uint32_t bz = 0; bz < size / 8; bz++)
+        for (uint32_t bx = 0; bx < size / 8; bx++)
+            for (uint32_t by = 0; by < floor_h / 8; by++) w.AddBrick(bx, by, bz, floorc);
+    Rng rng(seed);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t ex = rng.range(emin, emax), ey = rng.range(emin, emax), ez = rng.range(emin, emax);
+        const uint32_t x = rng.range(0, size - ex - 1), z = rng.range(0, size - ez - 1);
+        const uint32_t c = rgb332_to_4444(rng.range(1, 255));
+        for (uint32_t yy = floor_h; yy < floor_h + ey; yy++)
+            for (uint32_t zz = z; zz < z + ez; zz++)
+                for (uint32_t xx = x; xx < x + ex; xx++) w.Set(xx, yy, zz, c);
+    }
+    return VR_OK;
+}
+
+int vrw_scene_light_clusters(vrw_world* wp, uint32_t n, uint32_t nclusters, float sigma, uint32_t size,
+                             uint32_t ymin, uint32_t ymax, uint32_t seed, uint32_t mmin, uint32_t mmax)
+{
+    // n emissive voxels in nclusters Gaussian clusters (centres uniform over
+    // [0,size)^2 x [ymin, ymax]), placed in empty voxels only
+    if (!wp || nclusters == 0 || !(sigma > 0.0f) || size == 0 || ymax <= ymin || mmin == 0 || mmax < mmin ||
+        mmax > 255)
+        return VR_ERR_ARG;
+    vrhost::World& w = *reinterpret_cast<vrhost::World*>(wp);
+    Rng rng(seed);
+    std::vector<double> cx(nclusters), cy(nclusters), cz(nclusters);
+    for (uint32_t k = 0; k < nclusters; k++)
+        cx[k] = rng.uniform() * size, cz[k] = rng.uniform() * size, cy[k] = ymin + rng.uniform() * (ymax - ymin);
+    auto gauss = [&]() {
+        // Box-Muller
+        const double u1 = std::max(rng.uniform(), 1e-12), u2 = rng.uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    uint32_t placed = 0, tries = 0;
+    while (placed < n && tries < n * 64u) {
+        tries++;
+        const uint32_t k = placed % nclusters;
+        const double x = cx[k] + sigma * gauss(), y = cy[k] + sigma * gauss(), z = cz[k] + sigma * gauss();
+        if (x < 0 || z < 0 || x >= size || z >= size || y < ymin || y >= 1024) continue;
+        const uint32_t ix = (uint32_t)x, iy = (uint32_t)y, iz = (uint32_t)z;
+        if (w.Get(ix, iy, iz) != 0u) continue;
+        const uint32_t c = rgb332_to_4444(rng.range(1, 255)), m = rng.range(mmin, mmax);
+        w.Set(ix, iy, iz, (m << 16) | c);
+        placed++;
+    }
+    return placed == n ? VR_OK : VR_ERR_ARG;
+}
+
+int vrw_cluster_lights(const vrw_world* wp, uint32_t k, uint32_t iters, uint32_t seed, uint32_t* out)
+{
+    // k-means over light positions: k-means++ seeding, then Lloyd iterations
+    if (!wp || !out || k == 0) return VR_ERR_ARG;
+    const vrhost::World& w = *reinterpret_cast<const vrhost::World*>(wp);
+    const auto& L = w.GetLights();
+    const size_t n = L.size();
+    if (n == 0) return VR_OK;
+    std::vector<double> px(n), py(n), pz(n);
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t p = L[i].position;
+        px[i] = (double)(p % 1024u), pz[i] = (double)((p / 1024u) % 1024u), py[i] = (double)(p / (1024u * 1024u));
+    }
+    k = (uint32_t)std::min<size_t>(k, n);
+    Rng rng(seed);
+    std::vector<double> cx, cy, cz, d2(n, 1e300);
+    size_t first = (size_t)(rng.uniform() * n) % n;
+    cx.push_back(px[first]), cy.push_back(py[first]), cz.push_back(pz[first]);
+    while (cx.size() < k) {
+        double total = 0;
+        for (size_t i = 0; i < n; i++) {
+            const double dx = px[i] - cx.back(), dy = py[i] - cy.back(), dz = pz[i] - cz.back();
+            d2[i] = std::min(d2[i], dx * dx + dy * dy + dz * dz);
+            total += d2[i];
+        }
+        double r = rng.uniform() * total;
+        size_t pick = n - 1;
+        for (size_t i = 0; i < n; i++) {
+            r -= d2[i];
+            if (r <= 0) {
+                pick = i;
+                break;
+            }
+        }
+        cx.push_back(px[pick]), cy.push_back(py[pick]), cz.push_back(pz[pick]);
+    }
+    std::vector<double> sx(k), sy(k), sz(k);
+    std::vector<uint32_t> cnt(k);
+    for (uint32_t it = 0; it <= iters; it++) {
+        for (size_t i = 0; i < n; i++) {
+            double best = 1e300;
+            uint32_t bk = 0;
+            for (uint32_t c = 0; c < k; c++) {
+                const double dx = px[i] - cx[c], dy = py[i] - cy[c], dz = pz[i] - cz[c];
+                const double d = dx * dx + dy * dy + dz * dz;
+                if (d < best) best = d, bk = c;
+            }
+            out[i] = bk;
+        }
+        if (it == iters) break;
+        std::fill(sx.begin(), sx.end(), 0.0), std::fill(sy.begin(), sy.end(), 0.0), std::fill(sz.begin(), sz.end(), 0.0);
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (size_t i = 0; i < n; i++) sx[out[i]] += px[i], sy[out[i]] += py[i], sz[out[i]] += pz[i], cnt[out[i]]++;
+        for (uint32_t c = 0; c < k; c++)
+            if (cnt[c]) cx[c] = sx[c] / cnt[c], cy[c] = sy[c] / cnt[c], cz[c] = sz[c] / cnt[c];
+    }
+    return VR_OK;
+}
+
 void vrw_log_lut(float lo, float hi, float lut[256])
 {
     lut[0] = 0.0f;
@@ -200,7 +316,7 @@ int vrw_camera_path(uint32_t kind, uint32_t seed, uint32_t frame, uint32_t nfram
         double P[8][3];
         for (int k = 0; k < 8; k++) {
             const double x = 0.2 * size + rng.uniform() * 0.6 * size, z = 0.2 * size + rng.uniform() * 0.6 * size;
-            const double h = vrw_terrain_height((float)x, (float)z, scale, terrain_seed);
+            const double h = scale ? vrw_terrain_height((float)x, (float)z, scale, terrain_seed) : 0.0;
             P[k][0] = x, P[k][1] = h + (0.08 + 0.12 * rng.uniform()) * size, P[k][2] = z;
         }
         auto eval = [&](double t, double out[3]) {

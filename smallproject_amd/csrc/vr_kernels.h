@@ -34,7 +34,26 @@ struct FrameArgs {
     float4 Nl, Nr, Nt, Nb;
     uint32_t point, gfxexp, source, axisOnlyFace, doubleRadiance;
     int32_t shadowSteps;
+    // learned source: candidate slot j of pixel i at [j * W*H + i]
+    const uint32_t* candIdx;
+    const float* candPdf;
 };
+
+// XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs in
+// linear id order, and each XCD has its own L2.  Block b runs tile
+// (b % 8) * (n/8) + b/8, so each XCD renders one contiguous band of 16x16
+// tiles and the neighbour gathers of the reuse passes and the DDA's grid and
+// brick reads hit the L2 of the XCD that needs them.  The tail beyond a
+// multiple of 8 keeps its own index, so the map is a bijection.
+__device__ __forceinline__ void tile_of_block(int& tx, int& ty)
+{
+    const uint32_t nbx = gridDim.x, nb = gridDim.x * gridDim.y;
+    const uint32_t b = blockIdx.y * nbx + blockIdx.x;
+    const uint32_t full = nb & ~7u;
+    const uint32_t t = b < full ? (b & 7u) * (full >> 3) + (b >> 3) : b;
+    tx = (int)(t % nbx);
+    ty = (int)(t / nbx);
+}
 
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s);
 hipError_t launch_commit(const uint32_t* ids, const uint32_t* src, uint32_t* pool, uint32_t n, uint32_t cap,

@@ -15,9 +15,11 @@ namespace vr {
 
 __device__ __forceinline__ void tile_pixel(int& x, int& y)
 {
+    int tx, ty;
+    tile_of_block(tx, ty);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    x = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
-    y = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
+    x = tx * 16 + (wv & 1) * 8 + (lane & 7);
+    y = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
 }
 
 __device__ __forceinline__ Res load_res(const ResPlanes& r, size_t i)
@@ -244,9 +246,23 @@ __global__ __launch_bounds__(256) void k_initial(FrameArgs a)
         const f3 spo = f3{s.sp.x + 0.1f * s.N.x, s.sp.y + 0.1f * s.N.y, s.sp.z + 0.1f * s.N.z};
         const uint32_t nL = a.L.n;
         const uint32_t nC = nL < a.nCand ? nL : a.nCand;
+        const size_t plane = (size_t)a.W * a.H;
         for (uint32_t i = 0; i < nC; i++) {
             float pr;
-            const uint32_t li = pick_light(a.L, a.source, seed, pr);
+            uint32_t li;
+            if (a.source == 2u) {
+                // learned proposal: the slot's light and its source pdf
+                li = a.candIdx[i * plane + idx];
+                pr = a.candPdf[i * plane + idx];
+                if (li >= nL || !(pr > 0.0f)) {
+                    // a target the proposal could not place: weight 0
+                    res.M += 1u;
+                    (void)rnd(seed);
+                    continue;
+                }
+            } else {
+                li = pick_light(a.L, a.source, seed, pr);
+            }
             const float4 la = a.L.a[li], lb = a.L.b[li];
             f3 pos;
             float invPdf;

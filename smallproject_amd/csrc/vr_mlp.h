@@ -1,0 +1,61 @@
+// vr_mlp.h - learned light-cluster proposal: parameter layout, device views
+// and launchers (kernels in vr_mlp.hip).
+#pragma once
+#include "vr_kernels.h"
+
+namespace vr {
+
+// flat parameter layout (FP32 master copy): W1[64][9], b1[64], W2[64][64],
+// b2[64], W3[256][64], b3[256]
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + 64 * 9;
+constexpr int kOffW2 = kOffB1 + 64;
+constexpr int kOffB2 = kOffW2 + 64 * 64;
+constexpr int kOffW3 = kOffB2 + 64;
+constexpr int kOffB3 = kOffW3 + 256 * 64;
+constexpr int kMlpParams = kOffB3 + 256;  // 21440
+constexpr int kMaxClusters = 256;
+constexpr int kCandidates = 32;           // candidate slots written by the proposal
+
+// read-only view of the network and the light clusters
+struct MlpView {
+    const __bf16* W1b;   // [64][16]
+    const __bf16* W2b;   // [64][64]
+    const __bf16* W2Tb;  // [64][64]
+    const __bf16* W3b;   // [256][64]
+    const __bf16* W3Tb;  // [64][256]
+    const float* b1;
+    const float* b2;
+    const float* b3;
+    const uint32_t* clusterStart;   // [nclusters + 1] offsets into clusterLights
+    const uint32_t* clusterLights;  // light indices grouped by cluster
+    uint32_t nclusters;
+};
+
+struct MlpBuffers {
+    __bf16* W1b;
+    __bf16* W2b;
+    __bf16* W2Tb;
+    __bf16* W3b;
+    __bf16* W3Tb;
+};
+
+// face of a hit word for the network input: +x,+y,+z,-x,-y,-z = 0..5
+// (the voxel normal points against D on the hit axis)
+__device__ __forceinline__ int mlp_face(uint32_t hitw)
+{
+    const int side = (int)((hitw >> 24) & 3u);
+    return side + (((hitw >> 26) & 1u) ? 3 : 0);
+}
+
+hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint32_t* candIdx, float* candPdf, uint32_t salt,
+                           hipStream_t s);
+hipError_t launch_mlp_probs(const MlpView& m, const float* x, uint32_t B, float* out, hipStream_t s);
+hipError_t launch_mlp_grad(const MlpView& m, const float* x, const uint32_t* t, const float* w, uint32_t B,
+                           float* partial, float* loss, float* grad, hipStream_t s);
+hipError_t launch_adam(float* P, const float* g, float* mo, float* ve, float lr, uint32_t t, hipStream_t s);
+hipError_t launch_mlp_cast(const float* P, const MlpBuffers& b, hipStream_t s);
+hipError_t launch_train_gather(const FrameArgs& a, const uint32_t* clusterOf, uint32_t n, uint32_t stride,
+                               uint32_t offset, float* x, uint32_t* t, float* w, hipStream_t s);
+
+}  // namespace vr

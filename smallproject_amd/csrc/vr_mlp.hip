@@ -1,0 +1,546 @@
+// vr_mlp.hip - the learned light-cluster proposal: a 9-64-64-256 MLP
+// (inputs: shading position / 1024 and the 6-way one-hot voxel face;
+// ReLU; softmax over light clusters) evaluated and trained with BF16 MFMA
+// (v_mfma_f32_32x32x16_bf16, FP32 accumulate).
+//
+// Layout conventions.  Every product is written C = A * B^T with both
+// operands K-contiguous ("NT"), so each lane feeds the MFMA with one 16-byte
+// load per operand.  For v_mfma_f32_32x32x16_bf16 lane l supplies row l%32,
+// k = 8*(l/32)..+7 of A and of B^T, and holds C(8*(q/4) + 4*(l/32) + q%4,
+// l%32) in accumulator register q.
+//
+// Parameters: FP32 master copy (flat, same order as the oracle: W1[64][9],
+// b1, W2[64][64], b2, W3[256][64], b3) plus BF16 copies for the MFMAs:
+// W1b[64][16] (inputs zero-padded), W2b[64][64], W2Tb[64][64], W3b[256][64],
+// W3Tb[64][256].
+#include "vr_mlp.h"
+
+namespace vr {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ __bf16 to_bf16(float f)
+{
+    // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    const unsigned short s = (unsigned short)u;
+    __bf16 b;
+    __builtin_memcpy(&b, &s, 2);
+    return b;
+}
+
+__device__ __forceinline__ float from_bf16(__bf16 b)
+{
+    unsigned short s;
+    __builtin_memcpy(&s, &b, 2);
+    return __uint_as_float((uint32_t)s << 16);
+}
+
+// acc += A[32 x K] * Bt[32 x K]^T, rows of A at a (stride lda), rows of Bt
+// at b (stride ldb), K a multiple of 16
+__device__ __forceinline__ f32x16 mfma_nt(const __bf16* a, int lda, const __bf16* b, int ldb, int K, f32x16 acc)
+{
+    const int l = threadIdx.x & 63;
+    const int r = l & 31, kh = (l >> 5) << 3;
+    for (int k = 0; k < K; k += 16) {
+        const bf16x8 av = *reinterpret_cast<const bf16x8*>(a + r * lda + k + kh);
+        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(b + r * ldb + k + kh);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int acc_row(int q) { return ((q >> 2) << 3) + (((threadIdx.x & 63) >> 5) << 2) + (q & 3); }
+__device__ __forceinline__ int acc_col() { return threadIdx.x & 31; }
+
+__device__ __forceinline__ f32x16 zero16()
+{
+    f32x16 z;
+#pragma unroll
+    for (int q = 0; q < 16; q++) z[q] = 0.0f;
+    return z;
+}
+
+// LDS of one wave's 64-sample chunk (16-byte aligned rows for the operand loads)
+struct __attribute__((aligned(16))) ChunkLds {
+    __bf16 X[64 * 16];    // inputs, sample-major
+    __bf16 H1[64 * 64];   // relu(layer 1), sample-major
+    __bf16 H2[64 * 64];   // relu(layer 2), sample-major
+};
+
+// hidden layers of 64 samples: X (LDS) -> H1, H2 (LDS)
+__device__ void forward_hidden(const MlpView& m, ChunkLds& s)
+{
+    for (int mb = 0; mb < 2; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            f32x16 acc = mfma_nt(s.X + mb * 32 * 16, 16, m.W1b + nb * 32 * 16, 16, 16, zero16());
+            const int j = nb * 32 + acc_col();
+            const float bj = m.b1[j];
+#pragma unroll
+            for (int q = 0; q < 16; q++) s.H1[(mb * 32 + acc_row(q)) * 64 + j] = to_bf16(fmaxf(acc[q] + bj, 0.0f));
+        }
+    __syncthreads();
+    for (int mb = 0; mb < 2; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            f32x16 acc = mfma_nt(s.H1 + mb * 32 * 64, 64, m.W2b + nb * 32 * 64, 64, 64, zero16());
+            const int j = nb * 32 + acc_col();
+            const float bj = m.b2[j];
+#pragma unroll
+            for (int q = 0; q < 16; q++) s.H2[(mb * 32 + acc_row(q)) * 64 + j] = to_bf16(fmaxf(acc[q] + bj, 0.0f));
+        }
+    __syncthreads();
+}
+
+// logits of cluster block mb for sample block nb: lane holds sample
+// nb*32 + l%32, clusters mb*32 + acc_row(q); empty clusters get -inf
+__device__ __forceinline__ f32x16 logits_block(const MlpView& m, const ChunkLds& s, int mb, int nb)
+{
+    f32x16 acc = mfma_nt(m.W3b + mb * 32 * 64, 64, s.H2 + nb * 32 * 64, 64, 64, zero16());
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int c = mb * 32 + acc_row(q);
+        acc[q] = (c < (int)m.nclusters && m.clusterStart[c + 1] > m.clusterStart[c]) ? acc[q] + m.b3[c] : -INFINITY;
+    }
+    return acc;
+}
+
+// per-sample softmax statistics (max, sum of exp) for sample block nb,
+// combined over the lane pair (l, l^32) that holds one sample
+__device__ __forceinline__ void softmax_stats(const MlpView& m, const ChunkLds& s, int nb, float& M, float& S,
+                                              float& Slo)
+{
+    float mx = -INFINITY, sum = 0.0f;
+    for (int mb = 0; mb < 8; mb++) {
+        const f32x16 z = logits_block(m, s, mb, nb);
+        float bm = mx;
+#pragma unroll
+        for (int q = 0; q < 16; q++) bm = fmaxf(bm, z[q]);
+        if (bm == -INFINITY) continue;  // nothing unmasked yet
+        float bs = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 16; q++) bs += z[q] == -INFINITY ? 0.0f : __expf(z[q] - bm);
+        sum = (mx == -INFINITY ? 0.0f : sum * __expf(mx - bm)) + bs;
+        mx = bm;
+    }
+    const float omx = __shfl_xor(mx, 32), osum = __shfl_xor(sum, 32);
+    M = fmaxf(mx, omx);
+    const float mine = mx == -INFINITY ? 0.0f : sum * __expf(mx - M);
+    const float other = omx == -INFINITY ? 0.0f : osum * __expf(omx - M);
+    S = mine + other;
+    Slo = (threadIdx.x & 63) < 32 ? mine : other;  // mass held by the lower lane of the pair
+}
+
+// ---------------------------------------------------------------------------
+// k_proposal: per pixel, 32 stratified candidates (light index, source pdf)
+// from q(cluster | x) / |cluster|.  Candidate j's target is (j + u_j)/32 of
+// the pixel's cluster mass: one sorted walk of the CDF serves all 32, and the
+// RIS estimator stays unbiased with weight p-hat/q because the strata
+// together cover the distribution once.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32_t* __restrict__ candIdx,
+                                                  float* __restrict__ candPdf, uint32_t salt)
+{
+    __shared__ ChunkLds lds[4];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    ChunkLds& s = lds[wv];
+    // the wave's 64 pixels: an 8x8 tile (same map as the frame kernels)
+    int tx, ty;
+    tile_of_block(tx, ty);
+    auto pixel_of = [&](int p, int& x, int& y) {
+        x = tx * 16 + (wv & 1) * 8 + (p & 7);
+        y = ty * 16 + (wv >> 1) * 8 + (p >> 3);
+    };
+    {
+        int x, y;
+        pixel_of(lane, x, y);
+        float in[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) in[i] = 0.0f;
+        if (x < (int)a.W && y < (int)a.H) {
+            const size_t idx = (size_t)y * a.W + x;
+            const uint32_t hitw = a.cur.hitw[idx];
+            if ((hitw & 0xffffffu) != 0u) {
+                const f3 D = camera_ray(a.cam, x, y);
+                const float dist = a.cur.dist[idx];
+                in[0] = (D.x * dist + a.cam.E.x) * (1.0f / 1024.0f);
+                in[1] = (D.y * dist + a.cam.E.y) * (1.0f / 1024.0f);
+                in[2] = (D.z * dist + a.cam.E.z) * (1.0f / 1024.0f);
+                in[3 + mlp_face(hitw)] = 1.0f;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) s.X[lane * 16 + i] = to_bf16(in[i]);
+    }
+    __syncthreads();
+    forward_hidden(m, s);
+    for (int nb = 0; nb < 2; nb++) {
+        const int p = nb * 32 + (lane & 31);
+        int x, y;
+        pixel_of(p, x, y);
+        const bool valid = x < (int)a.W && y < (int)a.H;
+        const size_t idx = valid ? (size_t)y * a.W + x : 0;
+        float M, S, Slo;
+        softmax_stats(m, s, nb, M, S, Slo);
+        // both lanes of the pair draw the same 64 uniforms from the pixel's
+        // proposal stream (the shading stream stays untouched)
+        uint32_t rs = wang_hash((valid ? a.cur.seed[idx] : 0u) ^ salt);
+        const bool upper = lane >= 32;
+        // walk: lower lane owns targets in [0, Slo), upper lane [Slo, S)
+        int j = 0;
+        float t = 0.0f, v = 0.0f;
+        auto next_target = [&]() {
+            // advance to the next target this lane owns; returns false when done
+            while (j < 32) {
+                const float u = rnd(rs);
+                v = rnd(rs);
+                t = ((float)j + u) * (1.0f / 32.0f) * S;
+                if ((t < Slo) != upper) {
+                    if (upper) t -= Slo;
+                    return true;
+                }
+                j++;
+            }
+            return false;
+        };
+        bool more = next_target();
+        float cdf = 0.0f;
+        uint32_t lastC = 0xffffffffu;
+        float lastP = 0.0f;
+        const size_t plane = (size_t)a.W * a.H;
+        auto emit = [&](uint32_t c, float pq) {
+            const uint32_t c0 = m.clusterStart[c], cnt = m.clusterStart[c + 1] - c0;
+            uint32_t k = (uint32_t)f2i(v * (float)cnt);
+            k = k >= cnt ? cnt - 1 : k;
+            if (valid) {
+                candIdx[(size_t)j * plane + idx] = m.clusterLights[c0 + k];
+                candPdf[(size_t)j * plane + idx] = (pq / S) / (float)cnt;
+            }
+        };
+        for (int mb = 0; mb < 8; mb++) {
+            const f32x16 z = logits_block(m, s, mb, nb);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const float pq = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M);
+                const float next = cdf + pq;
+                const uint32_t c = (uint32_t)(mb * 32 + acc_row(q));
+                while (more && t < next && pq > 0.0f) {
+                    emit(c, pq);
+                    j++;
+                    more = next_target();
+                }
+                if (pq > 0.0f) lastC = c, lastP = pq;
+                cdf = next;
+            }
+        }
+        // rounding can leave targets just above the walked mass: they take
+        // the last cluster with mass; a lane without any mass marks them
+        // invalid (candidate skipped with weight 0)
+        while (more) {
+            if (lastC != 0xffffffffu) {
+                emit(lastC, lastP);
+            } else if (valid) {
+                candIdx[(size_t)j * plane + idx] = 0xffffffffu;
+                candPdf[(size_t)j * plane + idx] = 0.0f;
+            }
+            j++;
+            more = next_target();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// probabilities for explicit inputs (test and evaluation hook)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mlp_probs(MlpView m, const float* __restrict__ x, uint32_t B,
+                                                  float* __restrict__ out)
+{
+    __shared__ ChunkLds s;
+    const int lane = threadIdx.x & 63;
+    const uint32_t base = blockIdx.x * 64;
+    {
+        const uint32_t b = base + lane;
+#pragma unroll
+        for (int i = 0; i < 16; i++) s.X[lane * 16 + i] = to_bf16(b < B && i < 9 ? x[(size_t)b * 9 + i] : 0.0f);
+    }
+    __syncthreads();
+    forward_hidden(m, s);
+    for (int nb = 0; nb < 2; nb++) {
+        float M, S, Slo;
+        softmax_stats(m, s, nb, M, S, Slo);
+        const uint32_t b = base + nb * 32 + (lane & 31);
+        for (int mb = 0; mb < 8; mb++) {
+            const f32x16 z = logits_block(m, s, mb, nb);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int c = mb * 32 + acc_row(q);
+                if (b < B) out[(size_t)b * 256 + c] = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M) / S;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training: per 64-sample chunk, forward + softmax cross-entropy backward;
+// writes the chunk's gradient (NPARAM floats) and weighted loss
+// ---------------------------------------------------------------------------
+struct __attribute__((aligned(16))) GradLds {
+    ChunkLds f;
+    __bf16 XT[32 * 64];    // inputs, feature-major (rows 16..31 zero)
+    __bf16 H1T[64 * 64];   // feature-major copies for weight gradients
+    __bf16 H2T[64 * 64];
+    __bf16 dZT[256 * 64];  // dL/dz, cluster-major
+    __bf16 dZ[64 * 256];   // dL/dz, sample-major
+    __bf16 dH2[64 * 64];
+    __bf16 dH2T[64 * 64];
+    __bf16 dH1T[64 * 64];
+};
+
+__global__ __launch_bounds__(64) void k_mlp_grad(MlpView m, const float* __restrict__ x, const uint32_t* __restrict__ tgt,
+                                                 const float* __restrict__ wgt, uint32_t B, float invB,
+                                                 float* __restrict__ partial, float* __restrict__ lossOut)
+{
+    __shared__ GradLds s;
+    const int lane = threadIdx.x & 63;
+    const uint32_t base = blockIdx.x * 64;
+    float* g = partial + (size_t)blockIdx.x * kMlpParams;
+    {
+        const uint32_t b = base + lane;
+        for (int i = 0; i < 16; i++) {
+            const __bf16 v = to_bf16(b < B && i < 9 ? x[(size_t)b * 9 + i] : 0.0f);
+            s.f.X[lane * 16 + i] = v;
+            s.XT[i * 64 + lane] = v;
+            s.XT[(16 + i) * 64 + lane] = to_bf16(0.0f);
+        }
+    }
+    __syncthreads();
+    forward_hidden(m, s.f);
+    for (int i = 0; i < 64; i++) {
+        s.H1T[i * 64 + lane] = s.f.H1[lane * 64 + i];
+        s.H2T[i * 64 + lane] = s.f.H2[lane * 64 + i];
+    }
+    float loss = 0.0f;
+    for (int nb = 0; nb < 2; nb++) {
+        float M, S, Slo;
+        softmax_stats(m, s.f, nb, M, S, Slo);
+        const uint32_t b = base + nb * 32 + (lane & 31);
+        const float w = b < B ? wgt[b] * invB : 0.0f;
+        const uint32_t t = b < B ? tgt[b] : 0u;
+        for (int mb = 0; mb < 8; mb++) {
+            const f32x16 z = logits_block(m, s.f, mb, nb);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int c = mb * 32 + acc_row(q);
+                const float p = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M) / S;
+                const float d = w * (p - (c == (int)t ? 1.0f : 0.0f));
+                const int smp = nb * 32 + (lane & 31);
+                const __bf16 db = to_bf16(d);
+                s.dZT[c * 64 + smp] = db;
+                s.dZ[smp * 256 + c] = db;
+                if (c == (int)t && w > 0.0f) loss += -w * (z[q] - M - __logf(S));
+            }
+        }
+    }
+    __syncthreads();
+    // dW3 = dZ^T H2 and db3
+    for (int mb = 0; mb < 8; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            const f32x16 acc = mfma_nt(s.dZT + mb * 32 * 64, 64, s.H2T + nb * 32 * 64, 64, 64, zero16());
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                g[kOffW3 + (mb * 32 + acc_row(q)) * 64 + nb * 32 + acc_col()] = acc[q];
+        }
+    for (int c = lane; c < 256; c += 64) {
+        float sum = 0.0f;
+        for (int i = 0; i < 64; i++) sum += from_bf16(s.dZT[c * 64 + i]);
+        g[kOffB3 + c] = sum;
+    }
+    // dH2 = dZ W3, masked by relu'(layer 2)
+    for (int mb = 0; mb < 2; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            const f32x16 acc = mfma_nt(s.dZ + mb * 32 * 256, 256, m.W3Tb + nb * 32 * 256, 256, 256, zero16());
+            const int h = nb * 32 + acc_col();
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int smp = mb * 32 + acc_row(q);
+                const float d = from_bf16(s.f.H2[smp * 64 + h]) > 0.0f ? acc[q] : 0.0f;
+                const __bf16 db = to_bf16(d);
+                s.dH2[smp * 64 + h] = db;
+                s.dH2T[h * 64 + smp] = db;
+            }
+        }
+    __syncthreads();
+    // dW2 = dH2^T H1 and db2
+    for (int mb = 0; mb < 2; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            const f32x16 acc = mfma_nt(s.dH2T + mb * 32 * 64, 64, s.H1T + nb * 32 * 64, 64, 64, zero16());
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                g[kOffW2 + (mb * 32 + acc_row(q)) * 64 + nb * 32 + acc_col()] = acc[q];
+        }
+    {
+        float sum = 0.0f;
+        for (int i = 0; i < 64; i++) sum += from_bf16(s.dH2T[lane * 64 + i]);
+        g[kOffB2 + lane] = sum;
+    }
+    // dH1 = dH2 W2, masked by relu'(layer 1)
+    for (int mb = 0; mb < 2; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            const f32x16 acc = mfma_nt(s.dH2 + mb * 32 * 64, 64, m.W2Tb + nb * 32 * 64, 64, 64, zero16());
+            const int h = nb * 32 + acc_col();
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int smp = mb * 32 + acc_row(q);
+                const float d = from_bf16(s.f.H1[smp * 64 + h]) > 0.0f ? acc[q] : 0.0f;
+                s.dH1T[h * 64 + smp] = to_bf16(d);
+            }
+        }
+    __syncthreads();
+    // dW1 = dH1^T X (9 real input columns) and db1
+    for (int mb = 0; mb < 2; mb++) {
+        const f32x16 acc = mfma_nt(s.dH1T + mb * 32 * 64, 64, s.XT, 64, 64, zero16());
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int i = acc_col();
+            if (i < 9) g[kOffW1 + (mb * 32 + acc_row(q)) * 9 + i] = acc[q];
+        }
+    }
+    {
+        float sum = 0.0f;
+        for (int i = 0; i < 64; i++) sum += from_bf16(s.dH1T[lane * 64 + i]);
+        g[kOffB1 + lane] = sum;
+    }
+    // the chunk's loss (lower lanes hold the target terms)
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) lossOut[blockIdx.x] = loss;
+}
+
+// sum the chunk gradients: grad[p] = sum_c partial[c][p]
+__global__ __launch_bounds__(256) void k_grad_reduce(const float* __restrict__ partial, uint32_t chunks,
+                                                     float* __restrict__ grad)
+{
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= (uint32_t)kMlpParams) return;
+    float sum = 0.0f;
+    for (uint32_t c = 0; c < chunks; c++) sum += partial[(size_t)c * kMlpParams + p];
+    grad[p] = sum;
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) on the FP32 master copy
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ P, const float* __restrict__ g, float* __restrict__ mo,
+                                              float* __restrict__ ve, float lr, float b1, float b2, float eps,
+                                              float c1, float c2)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint32_t)kMlpParams) return;
+    const float gi = g[i];
+    const float mi = b1 * mo[i] + (1.0f - b1) * gi;
+    const float vi = b2 * ve[i] + (1.0f - b2) * gi * gi;
+    mo[i] = mi, ve[i] = vi;
+    P[i] = P[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 copies of the master weights
+__global__ __launch_bounds__(256) void k_mlp_cast(const float* __restrict__ P, __bf16* W1b, __bf16* W2b, __bf16* W2Tb,
+                                                  __bf16* W3b, __bf16* W3Tb)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < 64 * 16) {
+        const uint32_t o = i / 16, k = i % 16;
+        W1b[i] = to_bf16(k < 9 ? P[kOffW1 + o * 9 + k] : 0.0f);
+    }
+    if (i < 64 * 64) {
+        const uint32_t o = i / 64, k = i % 64;
+        W2b[i] = to_bf16(P[kOffW2 + i]);
+        W2Tb[k * 64 + o] = to_bf16(P[kOffW2 + i]);
+    }
+    if (i < 256 * 64) {
+        const uint32_t o = i / 64, k = i % 64;
+        W3b[i] = to_bf16(P[kOffW3 + i]);
+        W3Tb[k * 256 + o] = to_bf16(P[kOffW3 + i]);
+    }
+}
+
+// training samples from the last frame: every stride-th pixel (offset
+// rotating with the frame); target = cluster of the final reservoir's
+// light, weight 1 where the reservoir holds a visible sample (W > 0)
+__global__ __launch_bounds__(256) void k_train_gather(FrameArgs a, const uint32_t* __restrict__ clusterOf, uint32_t n,
+                                                      uint32_t stride, uint32_t offset, float* __restrict__ x,
+                                                      uint32_t* __restrict__ tgt, float* __restrict__ wgt)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t npx = a.W * a.H;
+    const uint32_t idx = (uint32_t)(((uint64_t)k * stride + offset) % npx);
+    const int px = (int)(idx % a.W), py = (int)(idx / a.W);
+    const uint32_t hitw = a.cur.hitw[idx];
+    float in[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t t = 0;
+    float w = 0.0f;
+    if ((hitw & 0xffffffu) != 0u) {
+        const f3 D = camera_ray(a.cam, px, py);
+        const float dist = a.cur.dist[idx];
+        in[0] = (D.x * dist + a.cam.E.x) * (1.0f / 1024.0f);
+        in[1] = (D.y * dist + a.cam.E.y) * (1.0f / 1024.0f);
+        in[2] = (D.z * dist + a.cam.E.z) * (1.0f / 1024.0f);
+        in[3 + mlp_face(hitw)] = 1.0f;
+        const float4 ra = a.A.a[idx];
+        const uint32_t light = __float_as_uint(a.A.b[idx].w);
+        if (ra.y > 0.0f && light < a.L.n) {
+            t = clusterOf[light];
+            w = 1.0f;
+        }
+    }
+    for (int i = 0; i < 9; i++) x[(size_t)k * 9 + i] = in[i];
+    tgt[k] = t;
+    wgt[k] = w;
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint32_t* candIdx, float* candPdf, uint32_t salt,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_proposal, dim3((a.W + 15) / 16, (a.H + 15) / 16), dim3(256), 0, s, a, m, candIdx, candPdf, salt);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_probs(const MlpView& m, const float* x, uint32_t B, float* out, hipStream_t s)
+{
+    if (B == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_probs, dim3((B + 63) / 64), dim3(64), 0, s, m, x, B, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_grad(const MlpView& m, const float* x, const uint32_t* t, const float* w, uint32_t B,
+                           float* partial, float* loss, float* grad, hipStream_t s)
+{
+    if (B == 0) return hipSuccess;
+    const uint32_t chunks = (B + 63) / 64;
+    hipLaunchKernelGGL(k_mlp_grad, dim3(chunks), dim3(64), 0, s, m, x, t, w, B, 1.0f / (float)B, partial, loss);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, partial, chunks, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* P, const float* g, float* mo, float* ve, float lr, uint32_t t, hipStream_t s)
+{
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, P, g, mo, ve, lr, b1, b2, 1e-8f, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_cast(const float* P, const MlpBuffers& b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mlp_cast, dim3(64), dim3(256), 0, s, P, b.W1b, b.W2b, b.W2Tb, b.W3b, b.W3Tb);
+    return hipGetLastError();
+}
+
+hipError_t launch_train_gather(const FrameArgs& a, const uint32_t* clusterOf, uint32_t n, uint32_t stride,
+                               uint32_t offset, float* x, uint32_t* t, float* w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, a, clusterOf, n, stride, offset, x, t, w);
+    return hipGetLastError();
+}
+
+}  // namespace vr

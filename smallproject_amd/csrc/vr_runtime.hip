@@ -13,6 +13,7 @@
 
 #include "vr_api.h"
 #include "vr_kernels.h"
+#include "vr_mlp.h"
 
 using namespace vr;
 
@@ -56,6 +57,31 @@ struct vr_ctx {
     bool committed = false;
     bool framed = false;
     std::string err;
+    // learned proposal (allocated on first use)
+    bool mlpReady = false;
+    float* mlpP = nullptr;       // FP32 master parameters
+    float* mlpM = nullptr;       // Adam first moment
+    float* mlpV = nullptr;       // Adam second moment
+    float* mlpG = nullptr;       // gradient
+    __bf16* mlpW1b = nullptr;
+    __bf16* mlpW2b = nullptr;
+    __bf16* mlpW2Tb = nullptr;
+    __bf16* mlpW3b = nullptr;
+    __bf16* mlpW3Tb = nullptr;
+    uint32_t adamStep = 0;
+    uint32_t* clusterStart = nullptr;   // [kMaxClusters + 1]
+    uint32_t* clusterLights = nullptr;  // [lights]
+    uint32_t* clusterOf = nullptr;      // [lights]
+    uint32_t nclusters = 0, clusterLightCap = 0, clusteredLights = 0;
+    uint32_t* candIdx = nullptr;  // [kCandidates][W*H]
+    float* candPdf = nullptr;
+    float* trainX = nullptr;      // training batch
+    uint32_t* trainT = nullptr;
+    float* trainW = nullptr;
+    float* partial = nullptr;     // per-chunk gradients
+    float* chunkLoss = nullptr;
+    uint32_t trainCap = 0;
+    Camera lastCam{};             // camera of the last rendered frame
 };
 
 namespace {
@@ -219,7 +245,73 @@ FrameArgs frame_args(vr_ctx* c, const vr_render_params* p)
     a.axisOnlyFace = (c->cfg.quirks & VR_QUIRK_AXIS_ONLY_FACE) ? 1u : 0u;
     a.doubleRadiance = (c->cfg.quirks & VR_QUIRK_DOUBLE_RADIANCE) ? 1u : 0u;
     a.shadowSteps = (c->cfg.quirks & VR_QUIRK_SHADOW_STEP_CAP) ? 128 : 999999;
+    a.candIdx = c->candIdx;
+    a.candPdf = c->candPdf;
     return a;
+}
+
+MlpView mlp_view(const vr_ctx* c)
+{
+    return MlpView{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb, c->mlpP + kOffB1, c->mlpP + kOffB2,
+                   c->mlpP + kOffB3, c->clusterStart, c->clusterLights, c->nclusters};
+}
+
+// allocate the network, optimiser state, clusters and candidate planes
+int mlp_alloc(vr_ctx* c)
+{
+    if (c->mlpReady) return VR_OK;
+    const size_t npx = (size_t)c->cfg.width * c->cfg.height;
+    auto ok = [](hipError_t e) { return e == hipSuccess; };
+    const bool good = ok(dalloc(&c->mlpP, kMlpParams)) && ok(dalloc(&c->mlpM, kMlpParams)) &&
+                      ok(dalloc(&c->mlpV, kMlpParams)) && ok(dalloc(&c->mlpG, kMlpParams)) &&
+                      ok(dalloc(&c->mlpW1b, 64 * 16)) && ok(dalloc(&c->mlpW2b, 64 * 64)) &&
+                      ok(dalloc(&c->mlpW2Tb, 64 * 64)) && ok(dalloc(&c->mlpW3b, 256 * 64)) &&
+                      ok(dalloc(&c->mlpW3Tb, 256 * 64)) && ok(dalloc(&c->clusterStart, kMaxClusters + 1)) &&
+                      ok(dalloc(&c->candIdx, npx * kCandidates)) && ok(dalloc(&c->candPdf, npx * kCandidates));
+    if (!good) return fail(c, VR_ERR_NOMEM, "learned proposal: device allocation failed");
+    HIPCHK(c, hipMemsetAsync(c->mlpP, 0, kMlpParams * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlpM, 0, kMlpParams * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlpV, 0, kMlpParams * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->clusterStart, 0, (kMaxClusters + 1) * 4, c->stream));
+    HIPCHK(c, launch_mlp_cast(c->mlpP, MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->mlpReady = true;
+    return VR_OK;
+}
+
+int train_alloc(vr_ctx* c, uint32_t n)
+{
+    if (n <= c->trainCap) return VR_OK;
+    dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
+    c->trainX = nullptr, c->trainT = nullptr, c->trainW = nullptr, c->partial = nullptr, c->chunkLoss = nullptr;
+    c->trainCap = 0;
+    const uint32_t chunks = (n + 63) / 64;
+    if (dalloc(&c->trainX, (size_t)n * 9) != hipSuccess || dalloc(&c->trainT, n) != hipSuccess ||
+        dalloc(&c->trainW, n) != hipSuccess || dalloc(&c->partial, (size_t)chunks * kMlpParams) != hipSuccess ||
+        dalloc(&c->chunkLoss, chunks) != hipSuccess)
+        return fail(c, VR_ERR_NOMEM, "learned proposal: training buffers");
+    c->trainCap = n;
+    return VR_OK;
+}
+
+// one Adam step from the batch in trainX/T/W; returns the batch loss
+int train_step(vr_ctx* c, uint32_t n, float lr, double* loss)
+{
+    const uint32_t chunks = (n + 63) / 64;
+    HIPCHK(c, launch_mlp_grad(mlp_view(c), c->trainX, c->trainT, c->trainW, n, c->partial, c->chunkLoss, c->mlpG,
+                              c->stream));
+    c->adamStep++;
+    HIPCHK(c, launch_adam(c->mlpP, c->mlpG, c->mlpM, c->mlpV, lr, c->adamStep, c->stream));
+    HIPCHK(c, launch_mlp_cast(c->mlpP, MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, c->stream));
+    if (loss) {
+        std::vector<float> l(chunks);
+        HIPCHK(c, hipMemcpyAsync(l.data(), c->chunkLoss, (size_t)chunks * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        double s = 0;
+        for (float v : l) s += v;
+        *loss = s;
+    }
+    return VR_OK;
 }
 
 }  // namespace
@@ -245,7 +337,7 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     *out = nullptr;
     if (cfg->width == 0 || cfg->height == 0 || cfg->width > 16384 || cfg->height > 16384) return VR_ERR_ARG;
     if (cfg->brick_capacity == 0 || cfg->brick_capacity > (1u << 22)) return VR_ERR_ARG;
-    if (cfg->source_pdf > VR_SOURCE_POWER || cfg->light_mode > VR_LIGHTS_POINT) return VR_ERR_ARG;
+    if (cfg->source_pdf > VR_SOURCE_LEARNED || cfg->light_mode > VR_LIGHTS_POINT) return VR_ERR_ARG;
     if (cfg->emit_lut[0] != 0.0f) return VR_ERR_ARG;
     vr_ctx* c = new (std::nothrow) vr_ctx();
     if (!c) return VR_ERR_NOMEM;
@@ -292,6 +384,11 @@ int vr_destroy(vr_ctx* c)
         dfree(c->hitw[i]), dfree(c->dist[i]), dfree(c->seed[i]);
         dfree(c->ra[i]), dfree(c->rb[i]), dfree(c->rm[i]);
     }
+    dfree(c->mlpP), dfree(c->mlpM), dfree(c->mlpV), dfree(c->mlpG);
+    dfree(c->mlpW1b), dfree(c->mlpW2b), dfree(c->mlpW2Tb), dfree(c->mlpW3b), dfree(c->mlpW3Tb);
+    dfree(c->clusterStart), dfree(c->clusterLights), dfree(c->clusterOf);
+    dfree(c->candIdx), dfree(c->candPdf);
+    dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
     for (int i = 0; i <= VR_NUM_STAGES; i++)
         for (int f = 0; f < kRing; f++)
             if (c->ev[f][i]) (void)hipEventDestroy(c->ev[f][i]);
@@ -381,6 +478,14 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
     if (p->spatialRadius == 0 && p->spatial && p->spatialTaps)
         return fail(c, VR_ERR_ARG, "vr_render_frame: spatialRadius must be >= 1");
     if (!finite_camera(p)) return fail(c, VR_ERR_ARG, "vr_render_frame: non-finite camera parameters");
+    const bool learned = c->cfg.source_pdf == VR_SOURCE_LEARNED;
+    if (learned) {
+        if (!c->mlpReady || c->nclusters == 0 || c->clusteredLights != c->nLights)
+            return fail(c, VR_ERR_STATE, "vr_render_frame: the learned source needs vr_set_light_clusters for the "
+                                         "current light list");
+        if (p->numberOfCandidates > (uint32_t)kCandidates)
+            return fail(c, VR_ERR_ARG, "vr_render_frame: the learned source draws at most 32 candidates");
+    }
     c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
     const FrameArgs a = frame_args(c, p);
     const int slot = (int)(c->frameNo % kRing);
@@ -390,11 +495,17 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
     }
     HIPCHK(c, hipEventRecord(c->ev[slot][0], c->stream));
     for (int s = 0; s < VR_NUM_STAGES; s++) {
+        // the proposal runs after the primary hits and is timed with the
+        // candidate stage
+        if (s == VR_STAGE_INITIAL && learned)
+            HIPCHK(c, launch_proposal(a, mlp_view(c), c->candIdx, c->candPdf, p->R0 * 0x9e3779b9u + 0x632be5abu,
+                                      c->stream));
         HIPCHK(c, launch_stage(s, a, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[slot][s + 1], c->stream));
     }
     c->pending[slot] = c->profiling;
     c->lastSlot = slot;
+    c->lastCam = a.cam;
     c->frameNo++;
     c->framed = true;
     return VR_OK;
@@ -503,6 +614,145 @@ int vr_trace_rays(vr_ctx* c, const float* O, const float* D, uint32_t n, uint32_
     (void)hipStreamSynchronize(c->stream);
     dfree(dO), dfree(dD), dfree(dd), dfree(dv), dfree(ds);
     return rc;
+}
+
+int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights, uint32_t nclusters)
+{
+    if (!c || (nlights > 0 && !clusterOf)) return VR_ERR_ARG;
+    if (nclusters == 0 || nclusters > (uint32_t)kMaxClusters)
+        return fail(c, VR_ERR_ARG, "vr_set_light_clusters: 1..256 clusters");
+    if (nlights != c->nLights)
+        return fail(c, VR_ERR_ARG, "vr_set_light_clusters: cluster list must cover the uploaded lights");
+    for (uint32_t i = 0; i < nlights; i++)
+        if (clusterOf[i] >= nclusters) return fail(c, VR_ERR_ARG, "vr_set_light_clusters: cluster id out of range");
+    const int rc = mlp_alloc(c);
+    if (rc != VR_OK) return rc;
+    // counting sort of the light indices by cluster
+    std::vector<uint32_t> start(kMaxClusters + 1, 0u), lights(nlights);
+    for (uint32_t i = 0; i < nlights; i++) start[clusterOf[i] + 1]++;
+    for (uint32_t k = 0; k < (uint32_t)kMaxClusters; k++) start[k + 1] += start[k];
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    for (uint32_t i = 0; i < nlights; i++) lights[fill[clusterOf[i]]++] = i;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (nlights > c->clusterLightCap) {
+        dfree(c->clusterLights), dfree(c->clusterOf);
+        c->clusterLights = nullptr, c->clusterOf = nullptr, c->clusterLightCap = 0;
+        if (dalloc(&c->clusterLights, nlights) != hipSuccess || dalloc(&c->clusterOf, nlights) != hipSuccess)
+            return fail(c, VR_ERR_NOMEM, "vr_set_light_clusters: allocation failed");
+        c->clusterLightCap = nlights;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->clusterStart, start.data(), (kMaxClusters + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    if (nlights) {
+        HIPCHK(c, hipMemcpyAsync(c->clusterLights, lights.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->clusterOf, clusterOf, (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nclusters = nclusters;
+    c->clusteredLights = nlights;
+    return VR_OK;
+}
+
+int vr_mlp_set_params(vr_ctx* c, const float* params, uint32_t n)
+{
+    if (!c || !params || n != (uint32_t)kMlpParams) return VR_ERR_ARG;
+    const int rc = mlp_alloc(c);
+    if (rc != VR_OK) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->mlpP, params, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlpM, 0, kMlpParams * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlpV, 0, kMlpParams * 4, c->stream));
+    c->adamStep = 0;
+    HIPCHK(c, launch_mlp_cast(c->mlpP, MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
+}
+
+int vr_mlp_get_params(vr_ctx* c, float* params, uint32_t n)
+{
+    if (!c || !params || n != (uint32_t)kMlpParams) return VR_ERR_ARG;
+    if (!c->mlpReady) return fail(c, VR_ERR_STATE, "vr_mlp_get_params: no network");
+    HIPCHK(c, hipMemcpyAsync(params, c->mlpP, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
+}
+
+int vr_mlp_probs(vr_ctx* c, const float* x, uint32_t B, float* q)
+{
+    if (!c || !x || !q || B == 0 || B > (1u << 22)) return VR_ERR_ARG;
+    if (!c->mlpReady || c->nclusters == 0) return fail(c, VR_ERR_STATE, "vr_mlp_probs: no network or clusters");
+    float *dx = nullptr, *dq = nullptr;
+    int rc = VR_OK;
+    if (dalloc(&dx, (size_t)B * 9) != hipSuccess || dalloc(&dq, (size_t)B * 256) != hipSuccess) {
+        rc = fail(c, VR_ERR_NOMEM, "vr_mlp_probs: allocation failed");
+    } else {
+        hipError_t e = hipMemcpyAsync(dx, x, (size_t)B * 36, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = launch_mlp_probs(mlp_view(c), dx, B, dq, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(q, dq, (size_t)B * 1024, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hipfail(c, e, "vr_mlp_probs");
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dfree(dx), dfree(dq);
+    return rc;
+}
+
+int vr_mlp_grad(vr_ctx* c, const float* x, const uint32_t* t, const float* w, uint32_t B, float* grad, double* loss)
+{
+    if (!c || !x || !t || !w || !grad || B == 0 || B > (1u << 20)) return VR_ERR_ARG;
+    if (!c->mlpReady || c->nclusters == 0) return fail(c, VR_ERR_STATE, "vr_mlp_grad: no network or clusters");
+    for (uint32_t i = 0; i < B; i++)
+        if (t[i] >= 256u) return fail(c, VR_ERR_ARG, "vr_mlp_grad: target out of range");
+    int rc = train_alloc(c, B);
+    if (rc != VR_OK) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->trainX, x, (size_t)B * 36, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->trainT, t, (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->trainW, w, (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
+    const uint32_t chunks = (B + 63) / 64;
+    HIPCHK(c, launch_mlp_grad(mlp_view(c), c->trainX, c->trainT, c->trainW, B, c->partial, c->chunkLoss, c->mlpG,
+                              c->stream));
+    HIPCHK(c, hipMemcpyAsync(grad, c->mlpG, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<float> l(chunks);
+    HIPCHK(c, hipMemcpyAsync(l.data(), c->chunkLoss, (size_t)chunks * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (loss) {
+        double s = 0;
+        for (float v : l) s += v;
+        *loss = s;
+    }
+    return VR_OK;
+}
+
+int vr_mlp_train(vr_ctx* c, uint32_t nsamples, float lr, uint32_t steps, double* loss)
+{
+    if (!c || nsamples == 0 || nsamples > (1u << 20) || steps == 0 || !(lr > 0.0f)) return VR_ERR_ARG;
+    if (!c->framed) return fail(c, VR_ERR_STATE, "vr_mlp_train: no rendered frame to learn from");
+    if (!c->mlpReady || c->nclusters == 0 || c->clusteredLights != c->nLights)
+        return fail(c, VR_ERR_STATE, "vr_mlp_train: clusters do not cover the current lights");
+    int rc = train_alloc(c, nsamples);
+    if (rc != VR_OK) return rc;
+    // samples from the last frame: the A planes hold its final reservoirs
+    FrameArgs a = frame_args(c, nullptr);
+    a.cam = c->lastCam;
+    a.L.n = c->nLights;
+    const uint32_t npx = c->cfg.width * c->cfg.height;
+    const uint32_t stride = npx / nsamples > 0 ? npx / nsamples : 1u;
+    const uint32_t offset = (uint32_t)((c->frameNo * 7919u) % stride);
+    HIPCHK(c, launch_train_gather(a, c->clusterOf, nsamples, stride, offset, c->trainX, c->trainT, c->trainW, c->stream));
+    for (uint32_t s = 0; s < steps; s++) {
+        rc = train_step(c, nsamples, lr, (s + 1 == steps) ? loss : nullptr);
+        if (rc != VR_OK) return rc;
+    }
+    return VR_OK;
+}
+
+int vr_read_candidates(vr_ctx* c, uint32_t* idx, float* pdf)
+{
+    if (!c) return VR_ERR_ARG;
+    if (!c->mlpReady) return fail(c, VR_ERR_STATE, "vr_read_candidates: no learned proposal");
+    const size_t n = (size_t)c->cfg.width * c->cfg.height * kCandidates;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (idx) HIPCHK(c, hipMemcpy(idx, c->candIdx, n * 4, hipMemcpyDeviceToHost));
+    if (pdf) HIPCHK(c, hipMemcpy(pdf, c->candPdf, n * 4, hipMemcpyDeviceToHost));
+    return VR_OK;
 }
 
 int vr_ground_truth(vr_ctx* c, const vr_render_params* p, uint32_t spp, uint32_t frame0, double* accum)
