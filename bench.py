@@ -94,6 +94,8 @@ def main() -> int:
     ap.add_argument("--config", default="c2_area_1024")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--tile-order", default="auto", choices=["auto", "linear", "xcd_rows", "xcd_cols"],
+                    help="workgroup->tile order; auto times each on the first frames after warm-up")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only the known-size copy used to scale HBM byte counters")
     args = ap.parse_args()
@@ -123,6 +125,14 @@ def main() -> int:
         step(f)
     r.sync()
     log(f"bench: {args.warmup} warm-up frames done")
+    from smallproject_amd.autotune import ORDERS, tune_tile_order
+    if args.tile_order == "auto":
+        tile_order, tune_ms = tune_tile_order(r, lambda i: step(args.warmup + i))
+        log("bench: tile orders " + "; ".join(f"{k} " + " ".join(f"{v:.4f}" for v in ms) for k, ms in tune_ms.items())
+            + f" -> {tile_order}")
+    else:
+        tile_order, tune_ms = args.tile_order, None
+        r.set_tile_order(ORDERS[tile_order])
     r.profile_begin()
     r.sync()
     t0 = time.perf_counter()
@@ -154,7 +164,10 @@ def main() -> int:
         "config": {"workload": f"{sc.name}: plain Voxel ReSTIR, area lights, {sc.width}x{sc.height}, "
                                f"1024^3 world, {len(w.lights())} emitters, M={sc.candidates}, "
                                f"{sc.taps} taps r={sc.radius}, temporal clamp {sc.temporal_clamp}x",
-                   "resolution": [sc.width, sc.height], "frames_in_path": sc.frames},
+                   "resolution": [sc.width, sc.height], "frames_in_path": sc.frames,
+                   "tile_order": tile_order,
+                   "tile_order_stage_ms": None if tune_ms is None else
+                   {k: [round(v, 4) for v in ms] for k, ms in tune_ms.items()}},
         "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -68,6 +68,12 @@ extern "C" {
 #define VR_STAGE_SHADE   3  /* renderRIS                cl/ris.cl:391-413 */
 #define VR_NUM_STAGES    4
 
+/* order in which workgroups take the 16x16 pixel tiles, per stage
+ * (vr_config.flags, or vr_set_tile_order); results do not depend on it */
+#define VR_TILES_LINEAR   0u  /* row-major */
+#define VR_TILES_XCD_ROWS 1u  /* each of the 8 XCDs takes one band of tile rows */
+#define VR_TILES_XCD_COLS 2u  /* each XCD takes one band of tile columns, row by row */
+
 /* struct Light, template/common.h:134-139 */
 typedef struct vr_light {
     uint32_t position; /* x + z*1024 + y*1024*1024 of the light's min corner */
@@ -97,7 +103,7 @@ typedef struct vr_config {
     uint32_t quirks;          /* VR_QUIRK_* */
     uint32_t source_pdf;      /* VR_SOURCE_* */
     uint32_t brick_capacity;  /* device brick pool size in bricks (reference: VR_BRICKCOUNT) */
-    uint32_t flags;           /* reserved, 0 */
+    uint32_t flags;           /* bits 2s..2s+1: tile order VR_TILES_* of stage s; bits 8+ zero */
     float emit_lut[256];      /* EmitStrength(v) = emit_lut[(v >> 16) & 255]; the
                                  reference's is emit_lut[m] = m (common.h:208-211) */
 } vr_config;
@@ -134,6 +140,11 @@ int vr_render_frame(vr_ctx* ctx, const vr_render_params* params);
 
 /* Block until all work of the context has finished. */
 int vr_sync(vr_ctx* ctx);
+
+/* Tile order VR_TILES_* of stage VR_STAGE_* (-1: every stage) for the
+ * following frames (a performance setting with no counterpart in the
+ * reference; outputs are identical for every order). */
+int vr_set_tile_order(vr_ctx* ctx, int32_t stage, uint32_t order);
 
 /* Read-backs (blocking), n = width*height.
  * hits (current frame): hitword = voxel | side<<24 | (D[side] > 0)<<26,
@@ -177,7 +188,10 @@ int vr_trace_rays(vr_ctx* ctx, const float* origins, const float* dirs, uint32_t
 /* cluster of every uploaded light (0..nclusters-1, nclusters <= 256); must
  * be called again after vr_set_lights */
 int vr_set_light_clusters(vr_ctx* ctx, const uint32_t* cluster_of_light, uint32_t nlights, uint32_t nclusters);
-/* FP32 master parameters in the flat layout above; setting them resets Adam */
+/* The network starts from a fixed He-uniform initialisation (seed 1, biases
+ * 0, output layer scaled by 1/8 so the first proposal is near-uniform over
+ * clusters).  FP32 master parameters in the flat layout above; setting them
+ * resets Adam */
 int vr_mlp_set_params(vr_ctx* ctx, const float* params, uint32_t n);
 int vr_mlp_get_params(vr_ctx* ctx, float* params, uint32_t n);
 /* cluster probabilities q[B][256] for inputs x[B][9] */
@@ -190,6 +204,9 @@ int vr_mlp_grad(vr_ctx* ctx, const float* x, const uint32_t* t, const float* w, 
  * rendered frame, then `steps` Adam steps (beta 0.9/0.999) at rate lr;
  * *loss (optional) receives the last step's loss */
 int vr_mlp_train(vr_ctx* ctx, uint32_t nsamples, float lr, uint32_t steps, double* loss);
+/* one Adam step on an explicit batch (x[B][9], cluster targets t[B],
+ * weights w[B]); *loss (optional) receives the batch loss before the step */
+int vr_mlp_step(vr_ctx* ctx, const float* x, const uint32_t* t, const float* w, uint32_t B, float lr, double* loss);
 /* the last frame's candidate slots: idx[32][W*H] light indices and pdf[32][W*H] */
 int vr_read_candidates(vr_ctx* ctx, uint32_t* idx, float* pdf);
 

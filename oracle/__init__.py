@@ -18,7 +18,7 @@ LIBPATH = HERE / "liboracle.so"
 TRACE_ALL, TRACE_OPAQUE, TRACE_THROUGH = 0, 1, 2
 QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
 QUIRKS_REFERENCE = 7
-SOURCE_UNIFORM, SOURCE_POWER = 0, 1
+SOURCE_UNIFORM, SOURCE_POWER, SOURCE_SLOTS = 0, 1, 2
 
 RAY_DTYPE = np.dtype([("voxel", "<u4"), ("dist", "<f4"), ("side", "<u4"), ("seed", "<u4"), ("D", "<f4", (3,))])
 RES_DTYPE = np.dtype([("wsum", "<f4"), ("M", "<u4"), ("lightIndex", "<u4"), ("W", "<f4"), ("pos", "<f4", (3,)),
@@ -41,7 +41,7 @@ class Params(C.Structure):
 class Config(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("point_lights", C.c_uint32),
                 ("gfxexp", C.c_uint32), ("quirks", C.c_uint32), ("source_pdf", C.c_uint32),
-                ("emit_lut", C.c_float * 256)]
+                ("emit_lut", C.c_float * 256), ("cand_idx", C.c_void_p), ("cand_pdf", C.c_void_p)]
 
 
 class WorldView(C.Structure):
@@ -163,6 +163,17 @@ def sincos(angle: float) -> tuple[float, float]:
     c = np.zeros(1, np.float32)
     lib().or_sincos(C.c_float(angle), ptr(s), ptr(c))
     return float(s[0]), float(c[0])
+
+
+def camera_rays(params: Params, width: int, height: int) -> np.ndarray:
+    """GenerateCameraRay directions of every pixel, [H*W, 3] float32."""
+    D = np.zeros((height * width, 3), np.float32)
+    out = np.zeros(3, np.float32)
+    for y in range(height):
+        for x in range(width):
+            lib().or_camera_ray(C.byref(params), x, y, ptr(out))
+            D[y * width + x] = out
+    return D
 
 
 # ---- worlds ---------------------------------------------------------------
@@ -298,7 +309,15 @@ class Frames:
             lib().or_state_destroy(self.h)
             self.h = None
 
+    def set_slots(self, idx: np.ndarray, pdf: np.ndarray):
+        """Candidate lists of the next frame (SOURCE_SLOTS): idx/pdf [32, W*H]."""
+        self.slot_idx = np.ascontiguousarray(idx, np.uint32)
+        self.slot_pdf = np.ascontiguousarray(pdf, np.float32)
+        assert self.slot_idx.size >= 32 * self.W * self.H and self.slot_pdf.size == self.slot_idx.size
+        self.cfg.cand_idx, self.cfg.cand_pdf = ptr(self.slot_idx), ptr(self.slot_pdf)
+
     def render(self, params: Params, threads: int = 8):
+        assert self.cfg.source_pdf != SOURCE_SLOTS or self.cfg.cand_idx, "SOURCE_SLOTS needs set_slots()"
         lib().or_render_frame(self.h, C.byref(self.cfg), C.byref(params), self.view.ref, ptr(self.lights),
                               C.byref(self.alias) if self.alias is not None else None, threads)
 

@@ -682,7 +682,18 @@ void or_stage_initial(const or_config* c, const or_params* p, const or_world_vie
                 const uint32_t nC = nL < p->numberOfCandidates ? nL : p->numberOfCandidates;
                 for (uint32_t i = 0; i < nC; i++) {
                     float pr;
-                    const uint32_t li = gen_candidate(c, src, nL, seed, &pr);
+                    uint32_t li;
+                    if (c->source_pdf == OR_SOURCE_SLOTS) {
+                        const size_t slot = (size_t)i * W * H + (size_t)y * W + x;
+                        li = c->cand_idx[slot], pr = c->cand_pdf[slot];
+                        if (li >= nL || !(pr > 0.0f)) {
+                            res.M += 1;
+                            (void)or_random_float(seed);
+                            continue;
+                        }
+                    } else {
+                        li = gen_candidate(c, src, nL, seed, &pr);
+                    }
                     const or_light* light = &lights[li];
                     float pos[3], invPdf, Nl[3];
                     point_on_voxel_light(c, light, spo, seed, pos, &invPdf, Nl);

@@ -49,6 +49,7 @@ extern "C" {
 /* source pdf for candidate generation */
 #define OR_SOURCE_UNIFORM 0u  /* cl/tools.cl:593-598, p = 1/N */
 #define OR_SOURCE_POWER   1u  /* alias table over lightContribution, cl/tools.cl:579-591 */
+#define OR_SOURCE_SLOTS   2u  /* per-pixel candidate lists given by the caller (learned proposal) */
 
 typedef struct or_f3 { float x, y, z; } or_f3;
 
@@ -100,6 +101,11 @@ typedef struct or_config {
     uint32_t quirks;             /* OR_QUIRK_* */
     uint32_t source_pdf;         /* OR_SOURCE_* */
     float emit_lut[256];         /* EmitStrength(v) = emit_lut[(v >> 16) & 255] */
+    /* OR_SOURCE_SLOTS: candidate i of pixel p is light cand_idx[i*W*H + p]
+     * with source pdf cand_pdf[i*W*H + p]; an index >= N or a pdf <= 0 is
+     * an empty slot (M += 1, one random number consumed, no update) */
+    const uint32_t* cand_idx;
+    const float* cand_pdf;
 } or_config;
 
 /* read-only view of a device world: top grid, brick pool, uber grid */

@@ -22,6 +22,7 @@ CANDIDATES = 32
 QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
 QUIRKS_REFERENCE = 7
 TRACE_ALL, TRACE_OPAQUE = 0, 1
+TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS = 0, 1, 2
 NUM_STAGES = 4
 STAGE_NAMES = ("primary", "initial", "spatial", "shade")
 GRIDSIZE = 128 ** 3
@@ -66,6 +67,7 @@ SIGNATURES = {
     "vr_set_lights": (C.c_int, [_P, _P, _u32]),
     "vr_render_frame": (C.c_int, [_P, _P]),
     "vr_sync": (C.c_int, [_P]),
+    "vr_set_tile_order": (C.c_int, [_P, _i32, _u32]),
     "vr_read_hits": (C.c_int, [_P, _P, _P, _P]),
     "vr_read_reservoirs": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "vr_read_frame": (C.c_int, [_P, _P]),
@@ -80,6 +82,7 @@ SIGNATURES = {
     "vr_mlp_probs": (C.c_int, [_P, _P, _u32, _P]),
     "vr_mlp_grad": (C.c_int, [_P, _P, _P, _P, _u32, _P, _P]),
     "vr_mlp_train": (C.c_int, [_P, _u32, _f32, _u32, _P]),
+    "vr_mlp_step": (C.c_int, [_P, _P, _P, _P, _u32, _f32, _P]),
     "vr_read_candidates": (C.c_int, [_P, _P, _P]),
     # vr_world.h
     "vrw_create": (C.c_int, [_P, _P]),

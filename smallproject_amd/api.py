@@ -85,6 +85,9 @@ class Renderer:
     def sync(self):
         self._chk(lib().vr_sync(self.h), "vr_sync")
 
+    def set_tile_order(self, order: int, stage: int = -1):
+        self._chk(lib().vr_set_tile_order(self.h, stage, order), "vr_set_tile_order")
+
     def hits(self):
         hw = np.zeros(self.n, np.uint32)
         d = np.zeros(self.n, np.float32)
@@ -149,6 +152,15 @@ class Renderer:
         loss = C.c_double()
         self._chk(lib().vr_mlp_grad(self.h, ptr(x), ptr(t), ptr(w), x.shape[0], ptr(g), C.byref(loss)), "vr_mlp_grad")
         return loss.value, g
+
+    def mlp_step(self, x: np.ndarray, t: np.ndarray, w: np.ndarray, lr: float = 1e-3) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(t, np.uint32)
+        w = np.ascontiguousarray(w, np.float32)
+        loss = C.c_double()
+        self._chk(lib().vr_mlp_step(self.h, ptr(x), ptr(t), ptr(w), x.shape[0], C.c_float(lr), C.byref(loss)),
+                  "vr_mlp_step")
+        return loss.value
 
     def mlp_train(self, nsamples: int = 1 << 16, lr: float = 1e-3, steps: int = 1) -> float:
         loss = C.c_double()

@@ -34,23 +34,36 @@ struct FrameArgs {
     float4 Nl, Nr, Nt, Nb;
     uint32_t point, gfxexp, source, axisOnlyFace, doubleRadiance;
     int32_t shadowSteps;
+    uint32_t tileOrder;  // VR_TILES_*
     // learned source: candidate slot j of pixel i at [j * W*H + i]
     const uint32_t* candIdx;
     const float* candPdf;
 };
 
-// XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs in
-// linear id order, and each XCD has its own L2.  Block b runs tile
-// (b % 8) * (n/8) + b/8, so each XCD renders one contiguous band of 16x16
-// tiles and the neighbour gathers of the reuse passes and the DDA's grid and
-// brick reads hit the L2 of the XCD that needs them.  The tail beyond a
-// multiple of 8 keeps its own index, so the map is a bijection.
-__device__ __forceinline__ void tile_of_block(int& tx, int& ty)
+// Tile order (block id -> 16x16 tile).  Workgroups are dealt round-robin to
+// the 8 XCDs in linear id order and each XCD has its own L2.
+//   VR_TILES_LINEAR   block b runs tile b (row-major): the tiles in flight
+//                     form one compact strip, neighbours on different XCDs
+//   VR_TILES_XCD_ROWS block b runs tile (b % 8) * (n/8) + b/8: each XCD
+//                     renders one contiguous band of rows (the tail beyond a
+//                     multiple of 8 keeps its own index)
+//   VR_TILES_XCD_COLS with 8 | tiles per row, XCD b % 8 renders column band
+//                     b % 8 row by row, so the tiles in flight still form
+//                     one strip and each XCD's part of it is contiguous;
+//                     other widths fall back to the linear order
+// Every order is a bijection on [0, n).
+__device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
 {
     const uint32_t nbx = gridDim.x, nb = gridDim.x * gridDim.y;
     const uint32_t b = blockIdx.y * nbx + blockIdx.x;
-    const uint32_t full = nb & ~7u;
-    const uint32_t t = b < full ? (b & 7u) * (full >> 3) + (b >> 3) : b;
+    uint32_t t = b;
+    if (order == 1u) {
+        const uint32_t full = nb & ~7u;
+        t = b < full ? (b & 7u) * (full >> 3) + (b >> 3) : b;
+    } else if (order == 2u && (nbx & 7u) == 0u) {
+        const uint32_t cw = nbx >> 3, i = b >> 3;
+        t = (i / cw) * nbx + (b & 7u) * cw + i % cw;
+    }
     tx = (int)(t % nbx);
     ty = (int)(t / nbx);
 }

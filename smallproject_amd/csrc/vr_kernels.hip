@@ -13,10 +13,10 @@
 
 namespace vr {
 
-__device__ __forceinline__ void tile_pixel(int& x, int& y)
+__device__ __forceinline__ void tile_pixel(uint32_t order, int& x, int& y)
 {
     int tx, ty;
-    tile_of_block(tx, ty);
+    tile_of_block(order, tx, ty);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     x = tx * 16 + (wv & 1) * 8 + (lane & 7);
     y = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
@@ -162,7 +162,7 @@ __device__ __forceinline__ uint32_t pick_light(const LightView& L, uint32_t sour
 __global__ __launch_bounds__(256) void k_primary(FrameArgs a)
 {
     int x, y;
-    tile_pixel(x, y);
+    tile_pixel(a.tileOrder, x, y);
     if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t seed = wang_hash((uint32_t)x * 171u + (uint32_t)y * 1773u + a.R0);
@@ -232,7 +232,7 @@ __device__ __forceinline__ Shading shading_at(const FrameArgs& a, int x, int y, 
 __global__ __launch_bounds__(256) void k_initial(FrameArgs a)
 {
     int x, y;
-    tile_pixel(x, y);
+    tile_pixel(a.tileOrder, x, y);
     if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t hitw = a.cur.hitw[idx];
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_initial(FrameArgs a)
 __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
 {
     int x, y;
-    tile_pixel(x, y);
+    tile_pixel(a.tileOrder, x, y);
     if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     Res res = load_res(a.B, idx);
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
 __global__ __launch_bounds__(256) void k_shade(FrameArgs a)
 {
     int x, y;
-    tile_pixel(x, y);
+    tile_pixel(a.tileOrder, x, y);
     if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t hitw = a.cur.hitw[idx];
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(WorldView w, const float* _
 __global__ __launch_bounds__(256) void k_ground_truth(FrameArgs a, uint32_t spp, uint32_t frame0, double* accum)
 {
     int x, y;
-    tile_pixel(x, y);
+    tile_pixel(a.tileOrder, x, y);
     if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     const f3 D = camera_ray(a.cam, x, y);

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32
     ChunkLds& s = lds[wv];
     // the wave's 64 pixels: an 8x8 tile (same map as the frame kernels)
     int tx, ty;
-    tile_of_block(tx, ty);
+    tile_of_block(a.tileOrder, tx, ty);
     auto pixel_of = [&](int p, int& x, int& y) {
         x = tx * 16 + (wv & 1) * 8 + (p & 7);
         y = ty * 16 + (wv >> 1) * 8 + (p >> 3);

@@ -245,6 +245,7 @@ FrameArgs frame_args(vr_ctx* c, const vr_render_params* p)
     a.axisOnlyFace = (c->cfg.quirks & VR_QUIRK_AXIS_ONLY_FACE) ? 1u : 0u;
     a.doubleRadiance = (c->cfg.quirks & VR_QUIRK_DOUBLE_RADIANCE) ? 1u : 0u;
     a.shadowSteps = (c->cfg.quirks & VR_QUIRK_SHADOW_STEP_CAP) ? 128 : 999999;
+    a.tileOrder = c->cfg.flags & 3u;
     a.candIdx = c->candIdx;
     a.candPdf = c->candPdf;
     return a;
@@ -269,7 +270,22 @@ int mlp_alloc(vr_ctx* c)
                       ok(dalloc(&c->mlpW3Tb, 256 * 64)) && ok(dalloc(&c->clusterStart, kMaxClusters + 1)) &&
                       ok(dalloc(&c->candIdx, npx * kCandidates)) && ok(dalloc(&c->candPdf, npx * kCandidates));
     if (!good) return fail(c, VR_ERR_NOMEM, "learned proposal: device allocation failed");
-    HIPCHK(c, hipMemsetAsync(c->mlpP, 0, kMlpParams * 4, c->stream));
+    // He-uniform weights (bound sqrt(6/fan_in)), zero biases; the output
+    // layer is scaled by 1/8 so the first proposals are near-uniform
+    std::vector<float> init(kMlpParams, 0.0f);
+    uint32_t rs = 1u;
+    auto uni = [&rs]() {
+        rs ^= rs << 13, rs ^= rs >> 17, rs ^= rs << 5;
+        return (float)((double)rs * (2.0 / 4294967296.0) - 1.0);
+    };
+    auto fill = [&](int off, int n, int fanIn, float scale) {
+        const float bound = scale * std::sqrt(6.0f / (float)fanIn);
+        for (int i = 0; i < n; i++) init[off + i] = bound * uni();
+    };
+    fill(kOffW1, 64 * 9, 9, 1.0f);
+    fill(kOffW2, 64 * 64, 64, 1.0f);
+    fill(kOffW3, 256 * 64, 64, 0.125f);
+    HIPCHK(c, hipMemcpyAsync(c->mlpP, init.data(), kMlpParams * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->mlpM, 0, kMlpParams * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->mlpV, 0, kMlpParams * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->clusterStart, 0, (kMaxClusters + 1) * 4, c->stream));
@@ -339,6 +355,9 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     if (cfg->brick_capacity == 0 || cfg->brick_capacity > (1u << 22)) return VR_ERR_ARG;
     if (cfg->source_pdf > VR_SOURCE_LEARNED || cfg->light_mode > VR_LIGHTS_POINT) return VR_ERR_ARG;
     if (cfg->emit_lut[0] != 0.0f) return VR_ERR_ARG;
+    if (cfg->flags > 0xffu) return VR_ERR_ARG;
+    for (int s = 0; s < VR_NUM_STAGES; s++)
+        if (((cfg->flags >> (2 * s)) & 3u) > VR_TILES_XCD_COLS) return VR_ERR_ARG;
     vr_ctx* c = new (std::nothrow) vr_ctx();
     if (!c) return VR_ERR_NOMEM;
     c->cfg = *cfg;
@@ -495,12 +514,14 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
     }
     HIPCHK(c, hipEventRecord(c->ev[slot][0], c->stream));
     for (int s = 0; s < VR_NUM_STAGES; s++) {
+        FrameArgs as = a;
+        as.tileOrder = (c->cfg.flags >> (2 * s)) & 3u;
         // the proposal runs after the primary hits and is timed with the
         // candidate stage
         if (s == VR_STAGE_INITIAL && learned)
-            HIPCHK(c, launch_proposal(a, mlp_view(c), c->candIdx, c->candPdf, p->R0 * 0x9e3779b9u + 0x632be5abu,
+            HIPCHK(c, launch_proposal(as, mlp_view(c), c->candIdx, c->candPdf, p->R0 * 0x9e3779b9u + 0x632be5abu,
                                       c->stream));
-        HIPCHK(c, launch_stage(s, a, c->stream));
+        HIPCHK(c, launch_stage(s, as, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[slot][s + 1], c->stream));
     }
     c->pending[slot] = c->profiling;
@@ -515,6 +536,16 @@ int vr_sync(vr_ctx* c)
 {
     if (!c) return VR_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
+}
+
+int vr_set_tile_order(vr_ctx* c, int32_t stage, uint32_t order)
+{
+    if (!c) return VR_ERR_ARG;
+    if (order > VR_TILES_XCD_COLS) return fail(c, VR_ERR_ARG, "vr_set_tile_order: unknown order");
+    if (stage < -1 || stage >= VR_NUM_STAGES) return fail(c, VR_ERR_ARG, "vr_set_tile_order: unknown stage");
+    for (int s = 0; s < VR_NUM_STAGES; s++)
+        if (stage < 0 || stage == s) c->cfg.flags = (c->cfg.flags & ~(3u << (2 * s))) | (order << (2 * s));
     return VR_OK;
 }
 
@@ -719,6 +750,22 @@ int vr_mlp_grad(vr_ctx* c, const float* x, const uint32_t* t, const float* w, ui
         *loss = s;
     }
     return VR_OK;
+}
+
+int vr_mlp_step(vr_ctx* c, const float* x, const uint32_t* t, const float* w, uint32_t B, float lr, double* loss)
+{
+    if (!c || !x || !t || !w || B == 0 || B > (1u << 20) || !(lr > 0.0f)) return VR_ERR_ARG;
+    if (!c->mlpReady || c->nclusters == 0) return fail(c, VR_ERR_STATE, "vr_mlp_step: no network or clusters");
+    for (uint32_t i = 0; i < B; i++)
+        if (t[i] >= 256u) return fail(c, VR_ERR_ARG, "vr_mlp_step: target out of range");
+    int rc = train_alloc(c, B);
+    if (rc != VR_OK) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->trainX, x, (size_t)B * 36, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->trainT, t, (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->trainW, w, (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
+    rc = train_step(c, B, lr, loss);
+    if (rc == VR_OK) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return rc;
 }
 
 int vr_mlp_train(vr_ctx* c, uint32_t nsamples, float lr, uint32_t steps, double* loss)

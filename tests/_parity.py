@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 
 import oracle as O
-from smallproject_amd import configs
+from smallproject_amd import _lib, configs
 
 # FP32 tolerance for float fields: |a - b| <= RTOL*|b| + ATOL on >= 99.9% of pixels
 RTOL, ATOL, PASS_FRACTION = 1e-5, 1e-7, 0.999
@@ -74,9 +74,14 @@ def params_for_oracle(p) -> O.Params:
 
 
 def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, height: int | None = None,
-               quirks: int | None = None, source: int | None = None, check_every: bool = True):
+               quirks: int | None = None, source: int | None = None, check_every: bool = True, setup=None):
     """Render `frames` frames of a configuration on the GPU and in the oracle
-    with identical per-frame params; returns a Comparison per checked frame."""
+    with identical per-frame params; returns a Comparison per checked frame.
+
+    With the learned source the oracle takes the GPU's candidate slots (light
+    index and source pdf per pixel and slot) as its candidate lists, so every
+    stage downstream of the proposal is compared exactly.  `setup(world,
+    renderer)` runs once after the world is on the device."""
     import dataclasses
     if quirks is not None or source is not None:
         sc = dataclasses.replace(sc, quirks=sc.quirks if quirks is None else quirks,
@@ -85,9 +90,13 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
     W, H = w.cfg.width, w.cfg.height
     w.force_sync()
     r = w.renderer
+    if setup is not None:
+        setup(w, r)
+    learned = sc.source == _lib.SOURCE_LEARNED
     view = O.View(w.grid(), w.bricks())
     lut = np.array(list(w.cfg.emit_lut), np.float32)
-    ocfg = O.make_config(W, H, sc.point, gfxexp=1, quirks=sc.quirks, source=sc.source, lut=lut)
+    ocfg = O.make_config(W, H, sc.point, gfxexp=1, quirks=sc.quirks,
+                         source=O.SOURCE_SLOTS if learned else sc.source, lut=lut)
     of = O.Frames(ocfg, view, w.lights())
     results = []
     for f in range(frames):
@@ -96,6 +105,8 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
         p = w.prepare_frame()
         w.finish_frame()
         r.render(p)
+        if learned:
+            of.set_slots(*r.candidates())
         of.render(params_for_oracle(p))
         if not (check_every or f == frames - 1):
             continue
