@@ -201,9 +201,12 @@ int vr_mlp_grad(vr_ctx* ctx, const float* x, const uint32_t* t, const float* w, 
                 double* loss);
 /* online training: gather nsamples (x, cluster of the final reservoir's
  * light, weight 1 if the reservoir holds a visible sample) from the last
- * rendered frame, then `steps` Adam steps (beta 0.9/0.999) at rate lr;
- * *loss (optional) receives the last step's loss */
-int vr_mlp_train(vr_ctx* ctx, uint32_t nsamples, float lr, uint32_t steps, double* loss);
+ * frame rendered by `from` (NULL: by ctx itself; otherwise a context on the
+ * same device with the same light list, e.g. one running the plain
+ * source), at hashed random pixels, then one Adam step (beta 0.9/0.999,
+ * rate lr) on each of `steps` equal minibatches of them; *loss (optional)
+ * receives the last minibatch's loss before its update */
+int vr_mlp_train(vr_ctx* ctx, const vr_ctx* from, uint32_t nsamples, float lr, uint32_t steps, double* loss);
 /* one Adam step on an explicit batch (x[B][9], cluster targets t[B],
  * weights w[B]); *loss (optional) receives the batch loss before the step */
 int vr_mlp_step(vr_ctx* ctx, const float* x, const uint32_t* t, const float* w, uint32_t B, float lr, double* loss);

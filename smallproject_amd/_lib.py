@@ -81,7 +81,7 @@ SIGNATURES = {
     "vr_mlp_get_params": (C.c_int, [_P, _P, _u32]),
     "vr_mlp_probs": (C.c_int, [_P, _P, _u32, _P]),
     "vr_mlp_grad": (C.c_int, [_P, _P, _P, _P, _u32, _P, _P]),
-    "vr_mlp_train": (C.c_int, [_P, _u32, _f32, _u32, _P]),
+    "vr_mlp_train": (C.c_int, [_P, _P, _u32, _f32, _u32, _P]),
     "vr_mlp_step": (C.c_int, [_P, _P, _P, _P, _u32, _f32, _P]),
     "vr_read_candidates": (C.c_int, [_P, _P, _P]),
     # vr_world.h

@@ -162,9 +162,12 @@ class Renderer:
                   "vr_mlp_step")
         return loss.value
 
-    def mlp_train(self, nsamples: int = 1 << 16, lr: float = 1e-3, steps: int = 1) -> float:
+    def mlp_train(self, nsamples: int = 1 << 16, lr: float = 1e-3, steps: int = 1, source=None) -> float:
+        """Online training step from the last frame of `source` (a Renderer
+        on the same light list, e.g. the plain-source one) or of this one."""
         loss = C.c_double()
-        self._chk(lib().vr_mlp_train(self.h, nsamples, C.c_float(lr), steps, C.byref(loss)), "vr_mlp_train")
+        src = None if source is None else source.h
+        self._chk(lib().vr_mlp_train(self.h, src, nsamples, C.c_float(lr), steps, C.byref(loss)), "vr_mlp_train")
         return loss.value
 
     def candidates(self):

@@ -69,74 +69,95 @@ __device__ __forceinline__ bool same_face(uint32_t a, uint32_t b, bool axis_only
     return ((a ^ b) & m) == 0u;
 }
 
-// RandomPointOnVoxel (cl/tools.cl:499-577): 12 draws for the six faces,
-// one for the face CDF.
-//
-// Face i has the axis normal N = s*e_a, so dot(N, -L) = -s*L_a exactly (the
-// two zero products only add signed zeros).  When the shading point is not
-// strictly in front of the face plane (s*R_a >= 0) that value is <= 0, or NaN
-// for R = 0, and max(0, .) makes the face's solid angle zero: the square root
-// and both divisions are skipped for those faces with identical results (a
-// zero's sign never reaches an output: it only enters sums and the 1/pdf
-// that is zeroed when infinite).
+// point of face i (normals +x,+y,+z,-x,-y,-z; tangent axis (a+1)%3,
+// bitangent (a+2)%3 with the face's sign, tools.cl:502-518): centre + hs*N -
+// hs*T - hs*B + r0*size*T + r1*size*B per axis, where the zero components of
+// N, T, B leave the other terms unchanged
+__device__ __forceinline__ f3 face_point(f3 c, float hs, float size, int a, float s, float r0, float r1)
+{
+    const float s0 = r0 * size, s1 = r1 * size;
+    const float pa = (a == 0 ? c.x : (a == 1 ? c.y : c.z)) + hs * s;
+    const float ct = a == 0 ? c.y : (a == 1 ? c.z : c.x);
+    const float cb = a == 0 ? c.z : (a == 1 ? c.x : c.y);
+    const float pt = (ct - hs * s) + s0 * s;
+    const float pb = (cb - hs * s) + s1 * s;
+    return a == 0 ? f3{pa, pt, pb} : (a == 1 ? f3{pb, pa, pt} : f3{pt, pb, pa});
+}
+
+// RandomPointOnVoxel (tools.cl:499-577): 12 draws for the six faces, one
+// for the face CDF.  Face i = (axis a, sign s) has normal N = s*e_a, so
+// dot(N, -L) = -s*L_a exactly, and its solid angle is non-zero only if
+// s*R[a] < 0 (otherwise max(0, .) clamps a value <= 0, or NaN for R = 0).
+// That holds for at most one face per axis (the +face iff fl(c+hs) < sp,
+// the -face iff fl(c-hs) > sp), so three evaluations replace six; a zero's
+// sign never reaches an output (it only enters sums and the 1/pdf that is
+// zeroed when infinite).  The six faces still draw their two numbers each,
+// in order, and the total and the CDF walk add the non-zero terms in face
+// order (x + 0 = x exactly), so every output is the six-face computation's.
 __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t& seed, f3& pos, float& invPdf,
                                                uint32_t& face)
 {
-    float solid[6];
-    f3 pts[6];
-    float total = 0.0f;
+    uint32_t u[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) u[k] = xorshift(seed);
+    const float r2 = rnd(seed);
     const float hs = 0.5f * size;
     const float area = size * size;
+    float sol[3];
+    int fidx[3];  // front face of each axis, or -1
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-        // normals +x,+y,+z,-x,-y,-z; tangent axis (a+1)%3, bitangent (a+2)%3
-        // with the face's sign (tools.cl:502-518)
-        const int a = i % 3, ta = (a + 1) % 3, ba = (a + 2) % 3;
-        const float s = i < 3 ? 1.0f : -1.0f;
-        const float r0 = rnd(seed), r1 = rnd(seed);
-        const float s0 = r0 * size, s1 = r1 * size;
-        // centre + hs*N - hs*T - hs*B + r0*size*T + r1*size*B per axis; the
-        // zero components of N, T, B leave the other terms unchanged
-        float p[3] = {c.x, c.y, c.z};
-        p[a] = p[a] + hs * s;
-        p[ta] = (p[ta] - hs * s) + s0 * s;
-        p[ba] = (p[ba] - hs * s) + s1 * s;
-        const f3 pt = f3{p[0], p[1], p[2]};
-        pts[i] = pt;
-        const f3 R = sub(pt, sp);
-        const float Ra = a == 0 ? R.x : (a == 1 ? R.y : R.z);
-        float sol = 0.0f;
-        if (s * Ra < 0.0f) {
-            const float d = dot(R, R);
-            const float inv = 1.0f / sqrtf(d);
-            const float ndl = -s * (Ra * inv);
-            const float d2 = fmaxf(1e-5f, d);
-            sol = fmaxf(0.0f, ndl * area / d2);
-        }
-        solid[i] = sol;
-        total += sol;
+    for (int a = 0; a < 3; a++) {
+        const float ca = a == 0 ? c.x : (a == 1 ? c.y : c.z);
+        const float spa = a == 0 ? sp.x : (a == 1 ? sp.y : sp.z);
+        const float Rp = (ca + hs) - spa, Rm = (ca - hs) - spa;
+        const bool plus = Rp < 0.0f, minus = Rm > 0.0f;
+        const float s = plus ? 1.0f : -1.0f;
+        const int i = plus ? a : a + 3;
+        const float r0 = (float)(plus ? u[2 * a] : u[2 * a + 6]) * 2.3283064365387e-10f;
+        const float r1 = (float)(plus ? u[2 * a + 1] : u[2 * a + 7]) * 2.3283064365387e-10f;
+        const f3 R = sub(face_point(c, hs, size, a, s, r0, r1), sp);
+        const float Ra = plus ? Rp : Rm;
+        const float d = dot(R, R);
+        const float inv = 1.0f / sqrtf(d);
+        const float ndl = -s * (Ra * inv);
+        const float d2 = fmaxf(1e-5f, d);
+        sol[a] = (plus || minus) ? fmaxf(0.0f, ndl * area / d2) : 0.0f;
+        fidx[a] = (plus || minus) ? i : -1;
     }
-    const float r2 = rnd(seed);
-    uint32_t index = 0;
-    float pdf = solid[0] / total;
-    float cdf = pdf;
-    if (cdf < r2) {
+    // face order: +x +y +z -x -y -z
+    float total = 0.0f;
 #pragma unroll
-        for (int i = 1; i < 6; i++) {
-            pdf = solid[i] / total;
-            cdf = cdf + pdf;
-            index = (uint32_t)i;
-            if (cdf >= r2) break;
+    for (int a = 0; a < 3; a++) total += fidx[a] == a ? sol[a] : 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) total += fidx[a] == a + 3 ? sol[a] : 0.0f;
+    int index = 5;
+    float pdf = 0.0f / total;  // face 5 when no front face reaches r2
+    if (total == 0.0f) {
+        index = 0;  // 0/0: the walk stops at face 0 with pdf NaN
+    } else {
+        float cdf = 0.0f;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const int a = k % 3;
+            if (!done && fidx[a] == k) {
+                const float p = sol[a] / total;
+                cdf = cdf + p;
+                if (cdf >= r2 || k == 5) index = k, pdf = p, done = cdf >= r2;
+            }
         }
     }
     float wgt = 1.0f / pdf;
     if (isinf(wgt) || isnan(wgt)) wgt = 0.0f;
     invPdf = wgt;
-    face = index;
-    pos = pts[0];
+    face = (uint32_t)index;
+    const int a = index % 3;
+    const float s = index < 3 ? 1.0f : -1.0f;
+    uint32_t w0 = u[0], w1 = u[1];
 #pragma unroll
-    for (int i = 1; i < 6; i++)
-        if (index == (uint32_t)i) pos = pts[i];
+    for (int k = 1; k < 6; k++)
+        if (index == k) w0 = u[2 * k], w1 = u[2 * k + 1];
+    pos = face_point(c, hs, size, a, s, (float)w0 * 2.3283064365387e-10f, (float)w1 * 2.3283064365387e-10f);
 }
 
 __device__ __forceinline__ uint32_t pick_light(const LightView& L, uint32_t source, uint32_t& seed, float& p)

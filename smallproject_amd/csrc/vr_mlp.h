@@ -55,7 +55,7 @@ hipError_t launch_mlp_grad(const MlpView& m, const float* x, const uint32_t* t, 
                            float* partial, float* loss, float* grad, hipStream_t s);
 hipError_t launch_adam(float* P, const float* g, float* mo, float* ve, float lr, uint32_t t, hipStream_t s);
 hipError_t launch_mlp_cast(const float* P, const MlpBuffers& b, hipStream_t s);
-hipError_t launch_train_gather(const FrameArgs& a, const uint32_t* clusterOf, uint32_t n, uint32_t stride,
-                               uint32_t offset, float* x, uint32_t* t, float* w, hipStream_t s);
+hipError_t launch_train_gather(const FrameArgs& a, const uint32_t* clusterOf, uint32_t n, uint32_t salt, float* x,
+                               uint32_t* t, float* w, hipStream_t s);
 
 }  // namespace vr

@@ -462,17 +462,18 @@ __global__ __launch_bounds__(256) void k_mlp_cast(const float* __restrict__ P, _
     }
 }
 
-// training samples from the last frame: every stride-th pixel (offset
-// rotating with the frame); target = cluster of the final reservoir's
+// training samples from the last frame: sample k reads the pixel
+// floor(hash(k ^ salt) * npx / 2^32), so any run of consecutive samples is
+// spread over the whole image; target = cluster of the final reservoir's
 // light, weight 1 where the reservoir holds a visible sample (W > 0)
 __global__ __launch_bounds__(256) void k_train_gather(FrameArgs a, const uint32_t* __restrict__ clusterOf, uint32_t n,
-                                                      uint32_t stride, uint32_t offset, float* __restrict__ x,
+                                                      uint32_t salt, float* __restrict__ x,
                                                       uint32_t* __restrict__ tgt, float* __restrict__ wgt)
 {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     const uint32_t npx = a.W * a.H;
-    const uint32_t idx = (uint32_t)(((uint64_t)k * stride + offset) % npx);
+    const uint32_t idx = (uint32_t)(((uint64_t)wang_hash(k ^ salt) * npx) >> 32);
     const int px = (int)(idx % a.W), py = (int)(idx / a.W);
     const uint32_t hitw = a.cur.hitw[idx];
     float in[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -536,10 +537,10 @@ hipError_t launch_mlp_cast(const float* P, const MlpBuffers& b, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_train_gather(const FrameArgs& a, const uint32_t* clusterOf, uint32_t n, uint32_t stride,
-                               uint32_t offset, float* x, uint32_t* t, float* w, hipStream_t s)
+hipError_t launch_train_gather(const FrameArgs& a, const uint32_t* clusterOf, uint32_t n, uint32_t salt, float* x,
+                               uint32_t* t, float* w, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, a, clusterOf, n, stride, offset, x, t, w);
+    hipLaunchKernelGGL(k_train_gather, dim3((n + 255) / 256), dim3(256), 0, s, a, clusterOf, n, salt, x, t, w);
     return hipGetLastError();
 }
 

@@ -205,7 +205,7 @@ bool finite_camera(const vr_render_params* p)
     return true;
 }
 
-FrameArgs frame_args(vr_ctx* c, const vr_render_params* p)
+FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
 {
     FrameArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -310,12 +310,13 @@ int train_alloc(vr_ctx* c, uint32_t n)
     return VR_OK;
 }
 
-// one Adam step from the batch in trainX/T/W; returns the batch loss
-int train_step(vr_ctx* c, uint32_t n, float lr, double* loss)
+// one Adam step from samples [first, first + n) of trainX/T/W; returns the
+// batch loss before the update
+int train_step(vr_ctx* c, uint32_t first, uint32_t n, float lr, double* loss)
 {
     const uint32_t chunks = (n + 63) / 64;
-    HIPCHK(c, launch_mlp_grad(mlp_view(c), c->trainX, c->trainT, c->trainW, n, c->partial, c->chunkLoss, c->mlpG,
-                              c->stream));
+    HIPCHK(c, launch_mlp_grad(mlp_view(c), c->trainX + (size_t)first * 9, c->trainT + first, c->trainW + first, n,
+                              c->partial, c->chunkLoss, c->mlpG, c->stream));
     c->adamStep++;
     HIPCHK(c, launch_adam(c->mlpP, c->mlpG, c->mlpM, c->mlpV, lr, c->adamStep, c->stream));
     HIPCHK(c, launch_mlp_cast(c->mlpP, MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, c->stream));
@@ -763,29 +764,39 @@ int vr_mlp_step(vr_ctx* c, const float* x, const uint32_t* t, const float* w, ui
     HIPCHK(c, hipMemcpyAsync(c->trainX, x, (size_t)B * 36, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->trainT, t, (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->trainW, w, (size_t)B * 4, hipMemcpyHostToDevice, c->stream));
-    rc = train_step(c, B, lr, loss);
+    rc = train_step(c, 0, B, lr, loss);
     if (rc == VR_OK) HIPCHK(c, hipStreamSynchronize(c->stream));
     return rc;
 }
 
-int vr_mlp_train(vr_ctx* c, uint32_t nsamples, float lr, uint32_t steps, double* loss)
+int vr_mlp_train(vr_ctx* c, const vr_ctx* from, uint32_t nsamples, float lr, uint32_t steps, double* loss)
 {
     if (!c || nsamples == 0 || nsamples > (1u << 20) || steps == 0 || !(lr > 0.0f)) return VR_ERR_ARG;
-    if (!c->framed) return fail(c, VR_ERR_STATE, "vr_mlp_train: no rendered frame to learn from");
+    const vr_ctx* src = from ? from : c;
+    if (!src->framed) return fail(c, VR_ERR_STATE, "vr_mlp_train: no rendered frame to learn from");
     if (!c->mlpReady || c->nclusters == 0 || c->clusteredLights != c->nLights)
         return fail(c, VR_ERR_STATE, "vr_mlp_train: clusters do not cover the current lights");
+    if (src->nLights != c->nLights)
+        return fail(c, VR_ERR_ARG, "vr_mlp_train: the sample context's light list differs from the clustered one");
     int rc = train_alloc(c, nsamples);
     if (rc != VR_OK) return rc;
+    if (src != c) {
+        // the sample context's frame must be complete before this stream reads it
+        const hipError_t e = hipStreamSynchronize(src->stream);
+        if (e != hipSuccess) return hipfail(c, e, "vr_mlp_train: sample context");
+    }
     // samples from the last frame: the A planes hold its final reservoirs
-    FrameArgs a = frame_args(c, nullptr);
-    a.cam = c->lastCam;
-    a.L.n = c->nLights;
-    const uint32_t npx = c->cfg.width * c->cfg.height;
-    const uint32_t stride = npx / nsamples > 0 ? npx / nsamples : 1u;
-    const uint32_t offset = (uint32_t)((c->frameNo * 7919u) % stride);
-    HIPCHK(c, launch_train_gather(a, c->clusterOf, nsamples, stride, offset, c->trainX, c->trainT, c->trainW, c->stream));
-    for (uint32_t s = 0; s < steps; s++) {
-        rc = train_step(c, nsamples, lr, (s + 1 == steps) ? loss : nullptr);
+    FrameArgs a = frame_args(src, nullptr);
+    a.cam = src->lastCam;
+    a.L.n = src->nLights;
+    HIPCHK(c, launch_train_gather(a, c->clusterOf, nsamples, (uint32_t)src->frameNo * 0x9e3779b9u + 0x7f4a7c15u,
+                                  c->trainX, c->trainT, c->trainW, c->stream));
+    // `steps` minibatches of consecutive samples (each spread over the image)
+    const uint32_t nb = steps < nsamples ? steps : nsamples;
+    for (uint32_t s = 0; s < nb; s++) {
+        const uint32_t first = (uint32_t)((uint64_t)nsamples * s / nb);
+        const uint32_t last = (uint32_t)((uint64_t)nsamples * (s + 1) / nb);
+        rc = train_step(c, first, last - first, lr, (s + 1 == nb) ? loss : nullptr);
         if (rc != VR_OK) return rc;
     }
     return VR_OK;

@@ -1,0 +1,193 @@
+"""Image quality: MSE against a converged binary64 reference, and the
+equal-time comparison of the learned light proposal with the plain
+power-proportional source.
+
+The reference measures image error with its MSE tool (mse.cpp:33-65) against
+ground-truth rasters written by the accumulating GT kernel
+(cl/kernels.cl:177-241, 304-326; mygame.cpp:518-549).  Here the converged
+image is vr_ground_truth (binary64 accumulation of render_di_gfxexp
+samples) and the error is vrw_image_error over the rgb channels.
+
+Radiance convention: the evaluation renders with the reference's reuse and
+shadow-ray rules but without its doubled radiance (renderRIS adds the
+opaque colour twice, cl/ris.cl:281-285), so that the ReSTIR image and the
+ground truth estimate the same quantity.
+
+Equal time.  Both methods render the same world, camera and frame count
+from reset temporal history; per-frame GPU time comes from the stage
+events (the learned method's proposal kernel is part of its candidate
+stage).  Before the comparison the network is trained online along the
+configuration's camera path, one training step per frame from the plain
+method's final reservoirs (2^16 samples, Adam lr 1e-3); during the
+comparison its weights are frozen.  Reported: the plain method's MSE after
+8 frames, the number of learned frames (and their time) needed to reach it,
+and the learned MSE at the plain method's 8-frame time.
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib
+from .api import image_error
+from .configs import CONFIGS, build_world
+
+EVAL_QUIRKS = _lib.QUIRK_AXIS_ONLY_FACE | _lib.QUIRK_SHADOW_STEP_CAP
+
+
+def _log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def camera_params(world, pos, target):
+    """The render params of a camera (advances the World's frame counters)."""
+    world.look_at(pos, target)
+    p = world.prepare_frame()
+    world.finish_frame()
+    return p
+
+
+def converged_reference(renderer, params, spp: int) -> tuple[np.ndarray, float]:
+    """Mean radiance [n, 3] over spp ground-truth samples (two independent
+    halves with different R0) and the reference's own noise, the expected
+    squared error of the mean: mean((g1 - g2)^2) / 4."""
+    half = max(1, spp // 2)
+    p1 = type(params).from_buffer_copy(params)
+    p2 = type(params).from_buffer_copy(params)
+    p2.R0 = (params.R0 ^ 0x9E3779B9) or 1
+    g1 = renderer.ground_truth(p1, half) / half
+    g2 = renderer.ground_truth(p2, half) / half
+    noise = float(np.mean((g1 - g2) ** 2) / 4.0)
+    return 0.5 * (g1 + g2), noise
+
+
+def mse(gt_rgb: np.ndarray, frame_rgba: np.ndarray, width: int, height: int) -> float:
+    gt = np.zeros((gt_rgb.shape[0], 4), np.float32)
+    gt[:, :3] = gt_rgb
+    return image_error(gt, frame_rgba, width, height)[0]
+
+
+def render_sequence(world, renderer, pos, target, frames: int, gt_rgb, width: int, height: int):
+    """`frames` frames at a fixed camera from reset temporal history: MSE
+    after each frame and GPU ms per frame."""
+    world.look_at(pos, target)
+    world.params.restirtemporalframe = 0
+    renderer.sync()
+    renderer.profile_begin()
+    errs = []
+    for _ in range(frames):
+        world.render()
+        errs.append(mse(gt_rgb, renderer.frame(), width, height))
+    s, n = renderer.profile_read()
+    return errs, float(s.sum()) / max(n, 1), [float(v) / max(n, 1) for v in s]
+
+
+def plain_mse(name: str, width=None, height=None, frames: int = 8, gt_spp: int = 4096, eval_frame=None) -> dict:
+    """MSE of the configuration's own (plain) method after `frames` frames
+    at one camera of its path."""
+    sc = dataclasses.replace(CONFIGS[name], quirks=EVAL_QUIRKS)
+    w = build_world(sc, width, height)
+    W, H = w.cfg.width, w.cfg.height
+    w.force_sync()
+    r = w.renderer
+    f = sc.frames // 2 if eval_frame is None else eval_frame
+    pos, tgt = sc.camera(f)
+    t0 = time.perf_counter()
+    gt, noise = converged_reference(r, camera_params(w, pos, tgt), gt_spp)
+    _log(f"evaluate: {name} converged reference ({gt_spp} spp) in {time.perf_counter() - t0:.1f} s")
+    errs, ms, stage = render_sequence(w, r, pos, tgt, frames, gt, W, H)
+    w.close()
+    return {"config": name, "resolution": [W, H], "camera_frame": f, "gt_spp": gt_spp, "gt_noise_mse": noise,
+            "frames": frames, "mse_per_frame": errs, "ms_per_frame": ms}
+
+
+def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_frames=None, eval_frames: int = 16,
+               gt_spp: int = 4096, nsamples: int = 1 << 16, lr: float = 1e-3, train_steps: int = 1,
+               nclusters: int = 256, eval_frame=None) -> dict:
+    base = CONFIGS[name]
+    scP = dataclasses.replace(base, source=_lib.SOURCE_POWER, quirks=EVAL_QUIRKS)
+    scL = dataclasses.replace(base, source=_lib.SOURCE_LEARNED, quirks=EVAL_QUIRKS)
+    wP = build_world(scP, width, height)
+    W, H = wP.cfg.width, wP.cfg.height
+    wP.force_sync()
+    rP = wP.renderer
+    wL = build_world(scL, width, height)
+    wL.force_sync()
+    rL = wL.renderer
+    clusters = wL.cluster_lights(min(nclusters, len(wL.lights())), 8, 1)
+    rL.set_light_clusters(clusters, int(clusters.max()) + 1)
+    # online training along the camera path from the plain method's frames
+    T = base.frames if train_frames is None else train_frames
+    losses = []
+    t0 = time.perf_counter()
+    for f in range(T):
+        pos, tgt = base.camera(f % base.frames)
+        wP.look_at(pos, tgt)
+        wP.render()
+        losses.append(rL.mlp_train(nsamples, lr, train_steps, source=rP))
+    rL.sync()
+    _log(f"evaluate: trained {T} frames x {train_steps} step(s) in {time.perf_counter() - t0:.1f} s, "
+         f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    f = base.frames // 2 if eval_frame is None else eval_frame
+    pos, tgt = base.camera(f)
+    t0 = time.perf_counter()
+    gt, noise = converged_reference(rP, camera_params(wP, pos, tgt), gt_spp)
+    _log(f"evaluate: converged reference ({gt_spp} spp) in {time.perf_counter() - t0:.1f} s, noise {noise:.3g}")
+    eP, msP, stP = render_sequence(wP, rP, pos, tgt, eval_frames, gt, W, H)
+    eL, msL, stL = render_sequence(wL, rL, pos, tgt, eval_frames, gt, W, H)
+    k_ref = min(8, eval_frames)
+    target = eP[k_ref - 1]
+    reach = next((k + 1 for k, e in enumerate(eL) if e <= target), None)
+    k_eq = max(1, min(eval_frames, int(k_ref * msP / msL)))
+    out = {
+        "config": name, "resolution": [W, H], "camera_frame": f, "gt_spp": gt_spp, "gt_noise_mse": noise,
+        "clusters": int(clusters.max()) + 1, "train_frames": T, "train_steps_per_frame": train_steps,
+        "train_samples": nsamples, "lr": lr, "train_loss_first": losses[0], "train_loss_last": losses[-1],
+        "plain": {"ms_per_frame": msP, "stage_ms": stP, "mse_per_frame": eP},
+        "learned": {"ms_per_frame": msL, "stage_ms": stL, "mse_per_frame": eL},
+        "plain_mse_at_8": target,
+        "learned_frames_to_reach": reach,
+        "time_ratio_to_reach": None if reach is None else reach * msL / (k_ref * msP),
+        "learned_mse_at_equal_time": eL[k_eq - 1], "equal_time_frames_learned": k_eq,
+        "mse_ratio_equal_time": eL[k_eq - 1] / target if target > 0 else None,
+    }
+    wP.close()
+    wL.close()
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--configs", default="c1_point_128,c2_area_1024,c3_clusters_1024")
+    ap.add_argument("--width", type=int)
+    ap.add_argument("--height", type=int)
+    ap.add_argument("--gt-spp", type=int, default=4096)
+    ap.add_argument("--train-frames", type=int)
+    ap.add_argument("--train-steps", type=int, default=1)
+    ap.add_argument("--eval-frames", type=int, default=16)
+    ap.add_argument("--json")
+    args = ap.parse_args(argv)
+    results = []
+    for name in args.configs.split(","):
+        t0 = time.perf_counter()
+        if CONFIGS[name].clusters:
+            res = equal_time(name, args.width, args.height, args.train_frames, args.eval_frames, args.gt_spp,
+                             train_steps=args.train_steps)
+        else:
+            res = plain_mse(name, args.width, args.height, 8, args.gt_spp)
+        res["seconds"] = round(time.perf_counter() - t0, 1)
+        print(json.dumps(res), flush=True)
+        results.append(res)
+    if args.json:
+        with open(args.json, "w") as fh:
+            json.dump(results, fh, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
