@@ -178,16 +178,26 @@ int vr_trace_rays(vr_ctx* ctx, const float* origins, const float* dirs, uint32_t
 
 /* ---- learned proposal (VR_SOURCE_LEARNED; no counterpart in the reference) ----
  * A 9-64-64-256 MLP maps a shading point (position / 1024, one-hot voxel
- * face) to a softmax over light clusters; a candidate is a cluster drawn
- * from it, then a light drawn uniformly inside the cluster, with source pdf
- * q(cluster)/|cluster| in the RIS weight p-hat/p.  The 32 candidates of a
- * pixel are stratified over the cluster distribution.  BF16 MFMA with FP32
- * accumulation for inference and training; FP32 master weights and Adam. */
+ * face) to a softmax q over light clusters.  A candidate is a cluster drawn
+ * from the defensive mixture m = (1 - eps) q + eps P (P_c = the cluster's
+ * share of the total light power), then a light drawn inside the cluster in
+ * proportion to its power; its source pdf m_c * pmf_i / P_c enters the RIS
+ * weight p-hat/p (eps = 1 is exactly the power source).  The 32 candidates
+ * of a pixel are stratified over m.  BF16 MFMA with FP32 accumulation for
+ * inference and training; FP32 master weights and Adam. */
 #define VR_MLP_PARAMS 21440u  /* W1[64][9] b1[64] W2[64][64] b2[64] W3[256][64] b3[256] */
 
 /* cluster of every uploaded light (0..nclusters-1, nclusters <= 256); must
  * be called again after vr_set_lights */
 int vr_set_light_clusters(vr_ctx* ctx, const uint32_t* cluster_of_light, uint32_t nlights, uint32_t nclusters);
+/* eps of the defensive mixture, 0..1 (default 0.1) */
+int vr_set_proposal_mix(vr_ctx* ctx, float eps);
+/* the cluster tables (any pointer may be NULL): start[257] offsets into
+ * lights[n] (light ids grouped by cluster), per-cluster power alias tables
+ * prob[n] / alias[n] (cluster-local indices) aligned with lights, share[n]
+ * = pmf_i / P_c by light id, power[256] = P_c */
+int vr_read_cluster_tables(vr_ctx* ctx, uint32_t* start, uint32_t* lights, float* prob, uint32_t* alias,
+                           float* share, float* power);
 /* The network starts from a fixed He-uniform initialisation (seed 1, biases
  * 0, output layer scaled by 1/8 so the first proposal is near-uniform over
  * clusters).  FP32 master parameters in the flat layout above; setting them
@@ -210,7 +220,8 @@ int vr_mlp_train(vr_ctx* ctx, const vr_ctx* from, uint32_t nsamples, float lr, u
 /* one Adam step on an explicit batch (x[B][9], cluster targets t[B],
  * weights w[B]); *loss (optional) receives the batch loss before the step */
 int vr_mlp_step(vr_ctx* ctx, const float* x, const uint32_t* t, const float* w, uint32_t B, float lr, double* loss);
-/* the last frame's candidate slots: idx[32][W*H] light indices and pdf[32][W*H] */
+/* the last frame's candidate slots: cluster[32][W*H] (0xffffffff = empty)
+ * and mixture mass[32][W*H] */
 int vr_read_candidates(vr_ctx* ctx, uint32_t* idx, float* pdf);
 
 /* FP64 converged reference image: render_di_gfxexp (cl/kernels.cl:177-241)

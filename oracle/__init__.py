@@ -41,7 +41,9 @@ class Params(C.Structure):
 class Config(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("point_lights", C.c_uint32),
                 ("gfxexp", C.c_uint32), ("quirks", C.c_uint32), ("source_pdf", C.c_uint32),
-                ("emit_lut", C.c_float * 256), ("cand_idx", C.c_void_p), ("cand_pdf", C.c_void_p)]
+                ("emit_lut", C.c_float * 256), ("cand_idx", C.c_void_p), ("cand_pdf", C.c_void_p),
+                ("cl_start", C.c_void_p), ("cl_lights", C.c_void_p), ("cl_prob", C.c_void_p),
+                ("cl_alias", C.c_void_p), ("cl_share", C.c_void_p)]
 
 
 class WorldView(C.Structure):
@@ -309,12 +311,23 @@ class Frames:
             lib().or_state_destroy(self.h)
             self.h = None
 
-    def set_slots(self, idx: np.ndarray, pdf: np.ndarray):
-        """Candidate lists of the next frame (SOURCE_SLOTS): idx/pdf [32, W*H]."""
+    def set_slots(self, idx: np.ndarray, pdf: np.ndarray, tables: dict):
+        """Candidate slots of the next frame (SOURCE_SLOTS): cluster and
+        cluster mass [32, W*H], and the cluster tables (start, lights, prob,
+        alias, share) they refer to."""
         self.slot_idx = np.ascontiguousarray(idx, np.uint32)
         self.slot_pdf = np.ascontiguousarray(pdf, np.float32)
         assert self.slot_idx.size >= 32 * self.W * self.H and self.slot_pdf.size == self.slot_idx.size
         self.cfg.cand_idx, self.cfg.cand_pdf = ptr(self.slot_idx), ptr(self.slot_pdf)
+        self.tables = {"start": np.ascontiguousarray(tables["start"], np.uint32),
+                       "lights": np.ascontiguousarray(tables["lights"], np.uint32),
+                       "prob": np.ascontiguousarray(tables["prob"], np.float32),
+                       "alias": np.ascontiguousarray(tables["alias"], np.uint32),
+                       "share": np.ascontiguousarray(tables["share"], np.float32)}
+        t = self.tables
+        assert t["start"].size == 257
+        self.cfg.cl_start, self.cfg.cl_lights, self.cfg.cl_prob = ptr(t["start"]), ptr(t["lights"]), ptr(t["prob"])
+        self.cfg.cl_alias, self.cfg.cl_share = ptr(t["alias"]), ptr(t["share"])
 
     def render(self, params: Params, threads: int = 8):
         assert self.cfg.source_pdf != SOURCE_SLOTS or self.cfg.cand_idx, "SOURCE_SLOTS needs set_slots()"

@@ -685,12 +685,22 @@ void or_stage_initial(const or_config* c, const or_params* p, const or_world_vie
                     uint32_t li;
                     if (c->source_pdf == OR_SOURCE_SLOTS) {
                         const size_t slot = (size_t)i * W * H + (size_t)y * W + x;
-                        li = c->cand_idx[slot], pr = c->cand_pdf[slot];
-                        if (li >= nL || !(pr > 0.0f)) {
+                        const uint32_t cl = c->cand_idx[slot];
+                        const float mc = c->cand_pdf[slot];
+                        const uint32_t c0 = cl < 256u ? c->cl_start[cl] : 0u;
+                        const uint32_t cnt = cl < 256u ? c->cl_start[cl + 1] - c0 : 0u;
+                        const float u = or_random_float(seed);
+                        if (cnt == 0 || !(mc > 0.0f)) {
                             res.M += 1;
-                            (void)or_random_float(seed);
                             continue;
                         }
+                        int32_t k = f2i(u * (float)cnt);
+                        if (k < 0) k = 0;
+                        if (k > (int32_t)cnt - 1) k = (int32_t)cnt - 1;
+                        const float fr = u * (float)cnt - (float)k;
+                        const uint32_t kk = fr < c->cl_prob[c0 + k] ? (uint32_t)k : c->cl_alias[c0 + k];
+                        li = c->cl_lights[c0 + kk];
+                        pr = mc * c->cl_share[li];
                     } else {
                         li = gen_candidate(c, src, nL, seed, &pr);
                     }

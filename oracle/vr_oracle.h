@@ -101,11 +101,20 @@ typedef struct or_config {
     uint32_t quirks;             /* OR_QUIRK_* */
     uint32_t source_pdf;         /* OR_SOURCE_* */
     float emit_lut[256];         /* EmitStrength(v) = emit_lut[(v >> 16) & 255] */
-    /* OR_SOURCE_SLOTS: candidate i of pixel p is light cand_idx[i*W*H + p]
-     * with source pdf cand_pdf[i*W*H + p]; an index >= N or a pdf <= 0 is
-     * an empty slot (M += 1, one random number consumed, no update) */
+    /* OR_SOURCE_SLOTS (learned proposal): candidate i of pixel p is a draw
+     * from cluster c = cand_idx[i*W*H + p] with cluster mass m =
+     * cand_pdf[i*W*H + p].  One random number u picks the light inside the
+     * cluster with its alias table (bucket k = (int)(u*cnt), kept when the
+     * fraction u*cnt - k < cl_prob[c0+k], else cl_alias[c0+k]); the source
+     * pdf of light i is m * cl_share[i].  An empty cluster or m <= 0 is an
+     * empty slot (M += 1, u consumed, no update). */
     const uint32_t* cand_idx;
     const float* cand_pdf;
+    const uint32_t* cl_start;   /* [257] offsets into cl_lights */
+    const uint32_t* cl_lights;  /* light ids grouped by cluster */
+    const float* cl_prob;       /* alias tables aligned with cl_lights */
+    const uint32_t* cl_alias;   /* cluster-local indices */
+    const float* cl_share;      /* by light id */
 } or_config;
 
 /* read-only view of a device world: top grid, brick pool, uber grid */

@@ -84,6 +84,8 @@ SIGNATURES = {
     "vr_mlp_train": (C.c_int, [_P, _P, _u32, _f32, _u32, _P]),
     "vr_mlp_step": (C.c_int, [_P, _P, _P, _P, _u32, _f32, _P]),
     "vr_read_candidates": (C.c_int, [_P, _P, _P]),
+    "vr_set_proposal_mix": (C.c_int, [_P, _f32]),
+    "vr_read_cluster_tables": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     # vr_world.h
     "vrw_create": (C.c_int, [_P, _P]),
     "vrw_destroy": (C.c_int, [_P]),

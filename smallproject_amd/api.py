@@ -170,7 +170,20 @@ class Renderer:
         self._chk(lib().vr_mlp_train(self.h, src, nsamples, C.c_float(lr), steps, C.byref(loss)), "vr_mlp_train")
         return loss.value
 
+    def set_proposal_mix(self, eps: float):
+        self._chk(lib().vr_set_proposal_mix(self.h, C.c_float(eps)), "vr_set_proposal_mix")
+
+    def cluster_tables(self, nlights: int) -> dict:
+        t = {"start": np.zeros(257, np.uint32), "lights": np.zeros(nlights, np.uint32),
+             "prob": np.zeros(nlights, np.float32), "alias": np.zeros(nlights, np.uint32),
+             "share": np.zeros(nlights, np.float32), "power": np.zeros(256, np.float32)}
+        self._chk(lib().vr_read_cluster_tables(self.h, ptr(t["start"]), ptr(t["lights"]), ptr(t["prob"]),
+                                               ptr(t["alias"]), ptr(t["share"]), ptr(t["power"])),
+                  "vr_read_cluster_tables")
+        return t
+
     def candidates(self):
+        """Last frame's learned slots: (cluster [32, n], mixture mass [32, n])."""
         idx = np.zeros((_lib.CANDIDATES, self.n), np.uint32)
         pdf = np.zeros((_lib.CANDIDATES, self.n), np.float32)
         self._chk(lib().vr_read_candidates(self.h, ptr(idx), ptr(pdf)), "vr_read_candidates")

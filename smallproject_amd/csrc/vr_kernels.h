@@ -35,9 +35,16 @@ struct FrameArgs {
     uint32_t point, gfxexp, source, axisOnlyFace, doubleRadiance;
     int32_t shadowSteps;
     uint32_t tileOrder;  // VR_TILES_*
-    // learned source: candidate slot j of pixel i at [j * W*H + i]
+    // learned source: candidate slot j of pixel i at [j * W*H + i] holds a
+    // cluster and its mixture mass; inside the cluster the light is drawn
+    // with the cluster's power alias table
     const uint32_t* candIdx;
     const float* candPdf;
+    const uint32_t* clusterStart;  // [257] offsets into clusterLights
+    const uint32_t* clusterLights; // light ids grouped by cluster
+    const float* clusterProb;      // alias table per cluster, aligned with clusterLights
+    const uint32_t* clusterAlias;  // (cluster-local indices)
+    const float* lightShare;       // [lights] pmf_i / P_c of the light's cluster
 };
 
 // Tile order (block id -> 16x16 tile).  Workgroups are dealt round-robin to

@@ -272,15 +272,25 @@ __global__ __launch_bounds__(256) void k_initial(FrameArgs a)
             float pr;
             uint32_t li;
             if (a.source == 2u) {
-                // learned proposal: the slot's light and its source pdf
-                li = a.candIdx[i * plane + idx];
-                pr = a.candPdf[i * plane + idx];
-                if (li >= nL || !(pr > 0.0f)) {
+                // learned proposal: the slot's cluster and mixture mass; the
+                // light inside the cluster by power (one draw for bucket and
+                // alias, as pick_light), pdf = m_c * pmf_i / P_c
+                const uint32_t c = a.candIdx[i * plane + idx];
+                const float mc = a.candPdf[i * plane + idx];
+                const uint32_t c0 = c < 256u ? a.clusterStart[c] : 0u;
+                const uint32_t cnt = c < 256u ? a.clusterStart[c + 1] - c0 : 0u;
+                const float u = rnd(seed);
+                if (cnt == 0u || !(mc > 0.0f)) {
                     // a target the proposal could not place: weight 0
                     res.M += 1u;
-                    (void)rnd(seed);
                     continue;
                 }
+                int32_t k = f2i(u * (float)cnt);
+                k = k < 0 ? 0 : (k > (int32_t)cnt - 1 ? (int32_t)cnt - 1 : k);
+                const float fr = u * (float)cnt - (float)k;
+                const uint32_t kk = fr < a.clusterProb[c0 + k] ? (uint32_t)k : a.clusterAlias[c0 + k];
+                li = a.clusterLights[c0 + kk];
+                pr = mc * a.lightShare[li];
             } else {
                 li = pick_light(a.L, a.source, seed, pr);
             }

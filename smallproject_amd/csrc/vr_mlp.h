@@ -27,9 +27,11 @@ struct MlpView {
     const float* b1;
     const float* b2;
     const float* b3;
-    const uint32_t* clusterStart;   // [nclusters + 1] offsets into clusterLights
-    const uint32_t* clusterLights;  // light indices grouped by cluster
+    const uint32_t* clusterStart;  // [kMaxClusters + 1] offsets into the cluster-ordered light list
+    const float* clusterPower;     // [kMaxClusters] P_c: the cluster's share of the total light power
     uint32_t nclusters;
+    float mix;                     // eps of the defensive mixture (1-eps) q + eps P
+    float powerLow;                // sum of P_c over the clusters with (c & 4) == 0
 };
 
 struct MlpBuffers {

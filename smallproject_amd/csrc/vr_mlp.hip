@@ -95,25 +95,27 @@ __device__ void forward_hidden(const MlpView& m, ChunkLds& s)
 
 // logits of cluster block mb for sample block nb: lane holds sample
 // nb*32 + l%32, clusters mb*32 + acc_row(q); empty clusters get -inf
-__device__ __forceinline__ f32x16 logits_block(const MlpView& m, const ChunkLds& s, int mb, int nb)
+// (cstart: the cluster offsets, in global memory or staged in LDS)
+__device__ __forceinline__ f32x16 logits_block(const MlpView& m, const uint32_t* cstart, const ChunkLds& s, int mb,
+                                               int nb)
 {
     f32x16 acc = mfma_nt(m.W3b + mb * 32 * 64, 64, s.H2 + nb * 32 * 64, 64, 64, zero16());
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int c = mb * 32 + acc_row(q);
-        acc[q] = (c < (int)m.nclusters && m.clusterStart[c + 1] > m.clusterStart[c]) ? acc[q] + m.b3[c] : -INFINITY;
+        acc[q] = (c < (int)m.nclusters && cstart[c + 1] > cstart[c]) ? acc[q] + m.b3[c] : -INFINITY;
     }
     return acc;
 }
 
 // per-sample softmax statistics (max, sum of exp) for sample block nb,
 // combined over the lane pair (l, l^32) that holds one sample
-__device__ __forceinline__ void softmax_stats(const MlpView& m, const ChunkLds& s, int nb, float& M, float& S,
-                                              float& Slo)
+__device__ __forceinline__ void softmax_stats(const MlpView& m, const uint32_t* cstart, const ChunkLds& s, int nb,
+                                              float& M, float& S, float& Slo)
 {
     float mx = -INFINITY, sum = 0.0f;
     for (int mb = 0; mb < 8; mb++) {
-        const f32x16 z = logits_block(m, s, mb, nb);
+        const f32x16 z = logits_block(m, cstart, s, mb, nb);
         float bm = mx;
 #pragma unroll
         for (int q = 0; q < 16; q++) bm = fmaxf(bm, z[q]);
@@ -133,18 +135,29 @@ __device__ __forceinline__ void softmax_stats(const MlpView& m, const ChunkLds& 
 }
 
 // ---------------------------------------------------------------------------
-// k_proposal: per pixel, 32 stratified candidates (light index, source pdf)
-// from q(cluster | x) / |cluster|.  Candidate j's target is (j + u_j)/32 of
-// the pixel's cluster mass: one sorted walk of the CDF serves all 32, and the
-// RIS estimator stays unbiased with weight p-hat/q because the strata
-// together cover the distribution once.
+// k_proposal: per pixel, 32 stratified cluster draws from the defensive
+// mixture m(c | x) = (1 - eps) q(c | x) + eps P_c, where q is the network's
+// softmax over the non-empty clusters and P_c the cluster's share of the
+// total light power.  Candidate j's target is (j + u_j)/32 of the mixture
+// mass: one sorted walk of the CDF serves all 32, and the strata together
+// cover the distribution once, so the RIS weight p-hat/p stays unbiased.
+// Slot j receives (cluster, m_c); the candidate pass draws the light inside
+// the cluster in proportion to its power, so the source pdf of light i is
+// m_c * pmf_i / P_c (eps = 1 gives the plain power source exactly).
+// The walk only writes; the cluster table is staged in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32_t* __restrict__ candIdx,
                                                   float* __restrict__ candPdf, uint32_t salt)
 {
     __shared__ ChunkLds lds[4];
+    __shared__ uint32_t sStart[kMaxClusters + 1];
+    __shared__ float sPower[kMaxClusters];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     ChunkLds& s = lds[wv];
+    for (int c = threadIdx.x; c <= kMaxClusters; c += 256) {
+        sStart[c] = m.clusterStart[c];
+        if (c < kMaxClusters) sPower[c] = m.clusterPower[c];
+    }
     // the wave's 64 pixels: an 8x8 tile (same map as the frame kernels)
     int tx, ty;
     tile_of_block(a.tileOrder, tx, ty);
@@ -175,6 +188,7 @@ __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32
     }
     __syncthreads();
     forward_hidden(m, s);
+    const float keep = 1.0f - m.mix;
     for (int nb = 0; nb < 2; nb++) {
         const int p = nb * 32 + (lane & 31);
         int x, y;
@@ -182,22 +196,23 @@ __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32
         const bool valid = x < (int)a.W && y < (int)a.H;
         const size_t idx = valid ? (size_t)y * a.W + x : 0;
         float M, S, Slo;
-        softmax_stats(m, s, nb, M, S, Slo);
+        softmax_stats(m, sStart, s, nb, M, S, Slo);
+        // mixture mass held by the lower lane (its clusters are those with (c & 4) == 0)
+        const float massLo = keep * (Slo / S) + m.mix * m.powerLow;
         // both lanes of the pair draw the same 64 uniforms from the pixel's
         // proposal stream (the shading stream stays untouched)
         uint32_t rs = wang_hash((valid ? a.cur.seed[idx] : 0u) ^ salt);
         const bool upper = lane >= 32;
-        // walk: lower lane owns targets in [0, Slo), upper lane [Slo, S)
+        // walk: lower lane owns targets in [0, massLo), upper lane [massLo, 1)
         int j = 0;
-        float t = 0.0f, v = 0.0f;
+        float t = 0.0f;
         auto next_target = [&]() {
             // advance to the next target this lane owns; returns false when done
             while (j < 32) {
                 const float u = rnd(rs);
-                v = rnd(rs);
-                t = ((float)j + u) * (1.0f / 32.0f) * S;
-                if ((t < Slo) != upper) {
-                    if (upper) t -= Slo;
+                t = ((float)j + u) * (1.0f / 32.0f);
+                if ((t < massLo) != upper) {
+                    if (upper) t -= massLo;
                     return true;
                 }
                 j++;
@@ -209,40 +224,33 @@ __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32
         uint32_t lastC = 0xffffffffu;
         float lastP = 0.0f;
         const size_t plane = (size_t)a.W * a.H;
-        auto emit = [&](uint32_t c, float pq) {
-            const uint32_t c0 = m.clusterStart[c], cnt = m.clusterStart[c + 1] - c0;
-            uint32_t k = (uint32_t)f2i(v * (float)cnt);
-            k = k >= cnt ? cnt - 1 : k;
-            if (valid) {
-                candIdx[(size_t)j * plane + idx] = m.clusterLights[c0 + k];
-                candPdf[(size_t)j * plane + idx] = (pq / S) / (float)cnt;
-            }
-        };
         for (int mb = 0; mb < 8; mb++) {
-            const f32x16 z = logits_block(m, s, mb, nb);
+            const f32x16 z = logits_block(m, sStart, s, mb, nb);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
-                const float pq = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M);
-                const float next = cdf + pq;
                 const uint32_t c = (uint32_t)(mb * 32 + acc_row(q));
-                while (more && t < next && pq > 0.0f) {
-                    emit(c, pq);
+                const float pq = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M);
+                const float mc = keep * (pq / S) + m.mix * sPower[c];
+                const float next = cdf + mc;
+                while (more && t < next && mc > 0.0f) {
+                    if (valid) {
+                        candIdx[(size_t)j * plane + idx] = c;
+                        candPdf[(size_t)j * plane + idx] = mc;
+                    }
                     j++;
                     more = next_target();
                 }
-                if (pq > 0.0f) lastC = c, lastP = pq;
+                if (mc > 0.0f) lastC = c, lastP = mc;
                 cdf = next;
             }
         }
         // rounding can leave targets just above the walked mass: they take
         // the last cluster with mass; a lane without any mass marks them
-        // invalid (candidate skipped with weight 0)
+        // empty (candidate skipped with weight 0)
         while (more) {
-            if (lastC != 0xffffffffu) {
-                emit(lastC, lastP);
-            } else if (valid) {
-                candIdx[(size_t)j * plane + idx] = 0xffffffffu;
-                candPdf[(size_t)j * plane + idx] = 0.0f;
+            if (valid) {
+                candIdx[(size_t)j * plane + idx] = lastC;
+                candPdf[(size_t)j * plane + idx] = lastP;
             }
             j++;
             more = next_target();
@@ -268,10 +276,10 @@ __global__ __launch_bounds__(64) void k_mlp_probs(MlpView m, const float* __rest
     forward_hidden(m, s);
     for (int nb = 0; nb < 2; nb++) {
         float M, S, Slo;
-        softmax_stats(m, s, nb, M, S, Slo);
+        softmax_stats(m, m.clusterStart, s, nb, M, S, Slo);
         const uint32_t b = base + nb * 32 + (lane & 31);
         for (int mb = 0; mb < 8; mb++) {
-            const f32x16 z = logits_block(m, s, mb, nb);
+            const f32x16 z = logits_block(m, m.clusterStart, s, mb, nb);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int c = mb * 32 + acc_row(q);
@@ -323,12 +331,12 @@ __global__ __launch_bounds__(64) void k_mlp_grad(MlpView m, const float* __restr
     float loss = 0.0f;
     for (int nb = 0; nb < 2; nb++) {
         float M, S, Slo;
-        softmax_stats(m, s.f, nb, M, S, Slo);
+        softmax_stats(m, m.clusterStart, s.f, nb, M, S, Slo);
         const uint32_t b = base + nb * 32 + (lane & 31);
         const float w = b < B ? wgt[b] * invB : 0.0f;
         const uint32_t t = b < B ? tgt[b] : 0u;
         for (int mb = 0; mb < 8; mb++) {
-            const f32x16 z = logits_block(m, s.f, mb, nb);
+            const f32x16 z = logits_block(m, m.clusterStart, s.f, mb, nb);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int c = mb * 32 + acc_row(q);

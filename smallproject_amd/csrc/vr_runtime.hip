@@ -70,8 +70,15 @@ struct vr_ctx {
     __bf16* mlpW3Tb = nullptr;
     uint32_t adamStep = 0;
     uint32_t* clusterStart = nullptr;   // [kMaxClusters + 1]
-    uint32_t* clusterLights = nullptr;  // [lights]
+    uint32_t* clusterLights = nullptr;  // [lights] light ids grouped by cluster
     uint32_t* clusterOf = nullptr;      // [lights]
+    float* clusterProb = nullptr;       // [lights] per-cluster power alias tables
+    uint32_t* clusterAlias = nullptr;   // [lights] (cluster-local indices)
+    float* lightShare = nullptr;        // [lights] pmf_i / P_c
+    float* clusterPower = nullptr;      // [kMaxClusters] P_c
+    float powerLow = 0.0f;              // sum of P_c over clusters with (c & 4) == 0
+    float mix = 0.1f;                   // defensive mixture weight of the power source
+    std::vector<vr_light> hostLights;   // the uploaded light list
     uint32_t nclusters = 0, clusterLightCap = 0, clusteredLights = 0;
     uint32_t* candIdx = nullptr;  // [kCandidates][W*H]
     float* candPdf = nullptr;
@@ -132,6 +139,43 @@ void light_record(const vr_config& cfg, const vr_light& l, float4& a, float4& b)
     float pf;
     std::memcpy(&pf, &packed, 4);
     b = make_float4(r, g, bl, pf);
+}
+
+// power of a light: rgb sum x emission x emitting area (binary64), the
+// weight of the power-proportional source
+double light_power(const vr_config& cfg, const vr_light& l)
+{
+    const double pi = 3.14159265358979323846;
+    const double r = (double)((float)((l.voxel >> 8) & 15u) * (1.0f / 15.0f));
+    const double g = (double)((float)((l.voxel >> 4) & 15u) * (1.0f / 15.0f));
+    const double b = (double)((float)(l.voxel & 15u) * (1.0f / 15.0f));
+    const double es = (double)cfg.emit_lut[(l.voxel >> 16) & 255u];
+    const double s = (double)l.size;
+    const double geo = cfg.light_mode == VR_LIGHTS_POINT ? 4.0 * pi : 6.0 * s * s * pi;
+    return (r + g + b) * es * geo;
+}
+
+// Vose alias table over n positive weights (binary64): bucket k keeps
+// itself with probability prob[k], else alias[k]
+void vose(const double* w, uint32_t n, float* prob, uint32_t* alias)
+{
+    double total = 0;
+    for (uint32_t i = 0; i < n; i++) total += w[i];
+    std::vector<double> sc(n);
+    std::vector<uint32_t> small, large;
+    for (uint32_t i = 0; i < n; i++) {
+        sc[i] = w[i] / total * (double)n;
+        (sc[i] < 1.0 ? small : large).push_back(i);
+        prob[i] = 1.0f, alias[i] = i;
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t s = small.back(), l = large.back();
+        small.pop_back(), large.pop_back();
+        prob[s] = (float)sc[s];
+        alias[s] = l;
+        sc[l] = (sc[l] + sc[s]) - 1.0;
+        (sc[l] < 1.0 ? small : large).push_back(l);
+    }
 }
 
 // Power-proportional source: Vose alias table over lightContribution
@@ -248,13 +292,18 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
     a.tileOrder = c->cfg.flags & 3u;
     a.candIdx = c->candIdx;
     a.candPdf = c->candPdf;
+    a.clusterStart = c->clusterStart;
+    a.clusterLights = c->clusterLights;
+    a.clusterProb = c->clusterProb;
+    a.clusterAlias = c->clusterAlias;
+    a.lightShare = c->lightShare;
     return a;
 }
 
 MlpView mlp_view(const vr_ctx* c)
 {
     return MlpView{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb, c->mlpP + kOffB1, c->mlpP + kOffB2,
-                   c->mlpP + kOffB3, c->clusterStart, c->clusterLights, c->nclusters};
+                   c->mlpP + kOffB3, c->clusterStart, c->clusterPower, c->nclusters, c->mix, c->powerLow};
 }
 
 // allocate the network, optimiser state, clusters and candidate planes
@@ -268,6 +317,7 @@ int mlp_alloc(vr_ctx* c)
                       ok(dalloc(&c->mlpW1b, 64 * 16)) && ok(dalloc(&c->mlpW2b, 64 * 64)) &&
                       ok(dalloc(&c->mlpW2Tb, 64 * 64)) && ok(dalloc(&c->mlpW3b, 256 * 64)) &&
                       ok(dalloc(&c->mlpW3Tb, 256 * 64)) && ok(dalloc(&c->clusterStart, kMaxClusters + 1)) &&
+                      ok(dalloc(&c->clusterPower, kMaxClusters)) &&
                       ok(dalloc(&c->candIdx, npx * kCandidates)) && ok(dalloc(&c->candPdf, npx * kCandidates));
     if (!good) return fail(c, VR_ERR_NOMEM, "learned proposal: device allocation failed");
     // He-uniform weights (bound sqrt(6/fan_in)), zero biases; the output
@@ -289,6 +339,7 @@ int mlp_alloc(vr_ctx* c)
     HIPCHK(c, hipMemsetAsync(c->mlpM, 0, kMlpParams * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->mlpV, 0, kMlpParams * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->clusterStart, 0, (kMaxClusters + 1) * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->clusterPower, 0, kMaxClusters * 4, c->stream));
     HIPCHK(c, launch_mlp_cast(c->mlpP, MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->mlpReady = true;
@@ -406,7 +457,8 @@ int vr_destroy(vr_ctx* c)
     }
     dfree(c->mlpP), dfree(c->mlpM), dfree(c->mlpV), dfree(c->mlpG);
     dfree(c->mlpW1b), dfree(c->mlpW2b), dfree(c->mlpW2Tb), dfree(c->mlpW3b), dfree(c->mlpW3Tb);
-    dfree(c->clusterStart), dfree(c->clusterLights), dfree(c->clusterOf);
+    dfree(c->clusterStart), dfree(c->clusterLights), dfree(c->clusterOf), dfree(c->clusterProb);
+    dfree(c->clusterAlias), dfree(c->lightShare), dfree(c->clusterPower);
     dfree(c->candIdx), dfree(c->candPdf);
     dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
     for (int i = 0; i <= VR_NUM_STAGES; i++)
@@ -472,6 +524,7 @@ int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
         c->lightCap = cap;
     }
     c->nLights = n;
+    c->hostLights.assign(lights, lights + n);
     if (n == 0) return VR_OK;
     HIPCHK(c, hipMemcpyAsync(c->la, ha.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->lb, hb.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
@@ -653,7 +706,7 @@ int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights
     if (!c || (nlights > 0 && !clusterOf)) return VR_ERR_ARG;
     if (nclusters == 0 || nclusters > (uint32_t)kMaxClusters)
         return fail(c, VR_ERR_ARG, "vr_set_light_clusters: 1..256 clusters");
-    if (nlights != c->nLights)
+    if (nlights != c->nLights || c->hostLights.size() != nlights)
         return fail(c, VR_ERR_ARG, "vr_set_light_clusters: cluster list must cover the uploaded lights");
     for (uint32_t i = 0; i < nlights; i++)
         if (clusterOf[i] >= nclusters) return fail(c, VR_ERR_ARG, "vr_set_light_clusters: cluster id out of range");
@@ -665,22 +718,78 @@ int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights
     for (uint32_t k = 0; k < (uint32_t)kMaxClusters; k++) start[k + 1] += start[k];
     std::vector<uint32_t> fill(start.begin(), start.end() - 1);
     for (uint32_t i = 0; i < nlights; i++) lights[fill[clusterOf[i]]++] = i;
+    // power of every light (the power source's pmf), each cluster's share
+    // P_c, the in-cluster share pmf_i / P_c and a power alias table per cluster
+    std::vector<double> pmf(nlights);
+    {
+        double total = 0;
+        for (uint32_t i = 0; i < nlights; i++) total += (pmf[i] = light_power(c->cfg, c->hostLights[i]));
+        for (uint32_t i = 0; i < nlights; i++) pmf[i] = total > 0 ? pmf[i] / total : 1.0 / nlights;
+    }
+    std::vector<float> power(kMaxClusters, 0.0f), share(nlights, 0.0f), prob(nlights, 1.0f);
+    std::vector<uint32_t> alias(nlights, 0u);
+    double low = 0;
+    for (uint32_t k = 0; k < (uint32_t)kMaxClusters; k++) {
+        const uint32_t c0 = start[k], cnt = start[k + 1] - c0;
+        if (cnt == 0) continue;
+        std::vector<double> w(cnt);
+        double pc = 0;
+        for (uint32_t j = 0; j < cnt; j++) pc += (w[j] = pmf[lights[c0 + j]]);
+        if (!(pc > 0)) std::fill(w.begin(), w.end(), 1.0);
+        for (uint32_t j = 0; j < cnt; j++)
+            share[lights[c0 + j]] = (float)(pc > 0 ? w[j] / pc : 1.0 / cnt);
+        vose(w.data(), cnt, prob.data() + c0, alias.data() + c0);
+        power[k] = (float)pc;
+        if ((k & 4u) == 0u) low += pc;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (nlights > c->clusterLightCap) {
-        dfree(c->clusterLights), dfree(c->clusterOf);
-        c->clusterLights = nullptr, c->clusterOf = nullptr, c->clusterLightCap = 0;
-        if (dalloc(&c->clusterLights, nlights) != hipSuccess || dalloc(&c->clusterOf, nlights) != hipSuccess)
+        dfree(c->clusterLights), dfree(c->clusterOf), dfree(c->clusterProb), dfree(c->clusterAlias), dfree(c->lightShare);
+        c->clusterLights = nullptr, c->clusterOf = nullptr, c->clusterProb = nullptr, c->clusterAlias = nullptr,
+        c->lightShare = nullptr, c->clusterLightCap = 0;
+        if (dalloc(&c->clusterLights, nlights) != hipSuccess || dalloc(&c->clusterOf, nlights) != hipSuccess ||
+            dalloc(&c->clusterProb, nlights) != hipSuccess || dalloc(&c->clusterAlias, nlights) != hipSuccess ||
+            dalloc(&c->lightShare, nlights) != hipSuccess)
             return fail(c, VR_ERR_NOMEM, "vr_set_light_clusters: allocation failed");
         c->clusterLightCap = nlights;
     }
     HIPCHK(c, hipMemcpyAsync(c->clusterStart, start.data(), (kMaxClusters + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->clusterPower, power.data(), kMaxClusters * 4, hipMemcpyHostToDevice, c->stream));
     if (nlights) {
         HIPCHK(c, hipMemcpyAsync(c->clusterLights, lights.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->clusterOf, clusterOf, (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->clusterProb, prob.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->clusterAlias, alias.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->lightShare, share.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->powerLow = (float)low;
     c->nclusters = nclusters;
     c->clusteredLights = nlights;
+    return VR_OK;
+}
+
+int vr_set_proposal_mix(vr_ctx* c, float eps)
+{
+    if (!c) return VR_ERR_ARG;
+    if (!(eps >= 0.0f && eps <= 1.0f)) return fail(c, VR_ERR_ARG, "vr_set_proposal_mix: eps must lie in [0, 1]");
+    c->mix = eps;
+    return VR_OK;
+}
+
+int vr_read_cluster_tables(vr_ctx* c, uint32_t* start, uint32_t* lights, float* prob, uint32_t* alias, float* share,
+                           float* power)
+{
+    if (!c) return VR_ERR_ARG;
+    if (!c->mlpReady || c->nclusters == 0) return fail(c, VR_ERR_STATE, "vr_read_cluster_tables: no clusters");
+    const size_t n = c->clusteredLights;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (start) HIPCHK(c, hipMemcpy(start, c->clusterStart, (kMaxClusters + 1) * 4, hipMemcpyDeviceToHost));
+    if (power) HIPCHK(c, hipMemcpy(power, c->clusterPower, kMaxClusters * 4, hipMemcpyDeviceToHost));
+    if (n && lights) HIPCHK(c, hipMemcpy(lights, c->clusterLights, n * 4, hipMemcpyDeviceToHost));
+    if (n && prob) HIPCHK(c, hipMemcpy(prob, c->clusterProb, n * 4, hipMemcpyDeviceToHost));
+    if (n && alias) HIPCHK(c, hipMemcpy(alias, c->clusterAlias, n * 4, hipMemcpyDeviceToHost));
+    if (n && share) HIPCHK(c, hipMemcpy(share, c->lightShare, n * 4, hipMemcpyDeviceToHost));
     return VR_OK;
 }
 

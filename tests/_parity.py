@@ -106,7 +106,7 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
         w.finish_frame()
         r.render(p)
         if learned:
-            of.set_slots(*r.candidates())
+            of.set_slots(*r.candidates(), r.cluster_tables(len(w.lights())))
         of.render(params_for_oracle(p))
         if not (check_every or f == frames - 1):
             continue
