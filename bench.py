@@ -9,7 +9,11 @@ DDA -> 32 RIS candidates + shadow ray + temporal reuse -> spatial reuse ->
 shading with a second shadow ray.  The world, lights and all per-pixel state
 are resident in HBM before the timed region starts.
 
-Prints one JSON line (see the README of the benchmark contract in DESIGN.md).
+Prints one JSON line: fps (value), Mrays/s, per-stage times, the dominant
+kernel's roofline (algorithmic HBM bytes over its event time, and the HBM
+traffic its committed counter profile measured), the equal-time MSE of the
+learned proposal against the plain source (quality) and the CPU oracle's
+frame rate on the host cores (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -86,6 +90,43 @@ def calibrate(launches: int = 10) -> int:
     return 0
 
 
+def measured_traffic(workload: str) -> dict:
+    """HBM bytes per launch of each stage kernel from the committed counter
+    profile of this workload (rocprofv3 FETCH_SIZE/WRITE_SIZE passes scaled
+    by a copy of known size), or {} when there is none."""
+    path = ROOT / "profiles" / f"hbm_traffic_{workload}.md"
+    if not path.exists():
+        return {}
+    for line in path.read_text().splitlines():
+        if line.strip().startswith("TRAFFIC_JSON "):
+            return json.loads(line.strip()[len("TRAFFIC_JSON "):])
+    return {}
+
+
+def quality(log_fn) -> dict:
+    """The metric's image-quality half: equal-time MSE against a 4096-spp
+    binary64 converged reference, learned proposal vs the plain power
+    source, on the many-light configuration at 1920x1080."""
+    from smallproject_amd.evaluate import equal_time
+    t0 = time.perf_counter()
+    r = equal_time("c3_clusters_1024")
+    log_fn(f"bench: equal-time MSE evaluation in {time.perf_counter() - t0:.1f} s")
+    return {"workload": "c3_clusters_1024: 30,000 boxes on a floor, 65,536 emitters in 256 clusters, 1920x1080, "
+                        "M=32, biased spatial reuse, static camera from reset history",
+            "reference": f"{r['gt_spp']}-spp binary64 ground truth (its own noise MSE {r['gt_noise_mse']:.2e})",
+            "mix": r["mix"], "train": f"{r['train_frames']} frames x {r['train_steps_per_frame']} Adam steps on 2^16 "
+                                      "samples/frame from the plain method's reservoirs",
+            "plain_ms_per_frame": round(r["plain"]["ms_per_frame"], 4),
+            "learned_ms_per_frame": round(r["learned"]["ms_per_frame"], 4),
+            "plain_mse_8_frames": r["plain_mse_at_8"],
+            "learned_frames_to_reach_it": r["learned_frames_to_reach"],
+            "time_ratio_to_reach_it": None if r["time_ratio_to_reach"] is None else round(r["time_ratio_to_reach"], 4),
+            "learned_mse_at_equal_time": r["learned_mse_at_equal_time"],
+            "mse_ratio_at_equal_time": None if r["mse_ratio_equal_time"] is None else round(r["mse_ratio_equal_time"], 4),
+            "plain_mse_per_frame": [round(v, 5) for v in r["plain"]["mse_per_frame"]],
+            "learned_mse_per_frame": [round(v, 5) for v in r["learned"]["mse_per_frame"]]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +134,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="c2_area_1024")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-quality", action="store_true", help="skip the equal-time MSE evaluation")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--tile-order", default="auto", choices=["auto", "linear", "xcd_rows", "xcd_cols"],
                     help="workgroup->tile order; auto times each on the first frames after warm-up")
@@ -148,6 +190,7 @@ def main() -> int:
     stage_ms = {name: float(stage_sum[i] / max(nprof, 1)) for i, name in enumerate(("primary", "initial", "spatial", "shade"))}
     dom = max(stage_ms, key=stage_ms.get)
     achieved = STAGE_BYTES[dom] * npx / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = measured_traffic(sc.name).get(f"k_{dom}")
     out = {
         "metric": METRIC,
         "value": round(fps, 3),
@@ -171,10 +214,13 @@ def main() -> int:
         "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "bytes_per_px": STAGE_BYTES[dom]},
+        "stage_hbm_gbs": {k: round(STAGE_BYTES[k] * npx / (v * 1e-3) / 1e9, 1) for k, v in stage_ms.items()},
     }
     w.close()
+    if not args.no_quality:
+        out["quality"] = quality(log)
     if not args.no_cpu_baseline:
         n = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sc, n)

@@ -190,7 +190,7 @@ int vr_trace_rays(vr_ctx* ctx, const float* origins, const float* dirs, uint32_t
 /* cluster of every uploaded light (0..nclusters-1, nclusters <= 256); must
  * be called again after vr_set_lights */
 int vr_set_light_clusters(vr_ctx* ctx, const uint32_t* cluster_of_light, uint32_t nlights, uint32_t nclusters);
-/* eps of the defensive mixture, 0..1 (default 0.1) */
+/* eps of the defensive mixture, 0..1 (default 0.25) */
 int vr_set_proposal_mix(vr_ctx* ctx, float eps);
 /* the cluster tables (any pointer may be NULL): start[257] offsets into
  * lights[n] (light ids grouped by cluster), per-cluster power alias tables

@@ -5,7 +5,8 @@ __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
 
 The reference's own implementation of this path (cl/*.cl, OpenCL C with
 image3d_t reads and work-item built-ins) needs an OpenCL runtime that this
-image does not have, so no oracle/_ref build exists; see DESIGN.md.
+image does not have (and no CPU OpenCL device), so no oracle/_ref build
+exists; the oracle is pinned by known answers and closed forms instead.
 """
 from __future__ import annotations
 

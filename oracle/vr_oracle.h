@@ -14,7 +14,8 @@
  * reference itself compiles with -cl-fast-relaxed-math, whose float bits are
  * implementation-defined; this restatement pins one exact definition.
  *
- * Parity status per function is listed in DESIGN.md ("oracle pinning").
+ * Each function is pinned by the tests named in its comment or in
+ * tests/test_oracle_*.py (known answers, closed forms, brute force).
  */
 #ifndef VR_ORACLE_H
 #define VR_ORACLE_H

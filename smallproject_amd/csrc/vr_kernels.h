@@ -26,6 +26,7 @@ struct FrameArgs {
     const float* prevDist;
     ResPlanes A;  // final reservoirs: previous frame's on entry, this frame's on exit
     ResPlanes B;  // candidates + temporal output
+    uint4* nbr;   // [2 per pixel] the B reservoir as the spatial pass's neighbours read it
     float4* frame;
     const float* emitLut;
     uint32_t W, H;

@@ -42,6 +42,17 @@ __device__ __forceinline__ void store_res(const ResPlanes& r, size_t i, const Re
     r.m[i] = (o.M & 0xffffffu) | (o.face << 24);
 }
 
+// The spatial pass's view of a neighbour, written by k_initial after its
+// reservoir is final: {M | face << 24 | hit side bits << 27, dist, W,
+// invPdf} {pos.xyz, light}.  A tap then reads one 32-byte record instead of
+// the hit word, distance and three reservoir planes.
+__device__ __forceinline__ void store_neighbour(uint4* nbr, size_t i, const Res& r, uint32_t hitw, float dist)
+{
+    nbr[2 * i] = make_uint4((r.M & 0xffffffu) | ((r.face & 7u) << 24) | (((hitw >> 24) & 7u) << 27),
+                            __float_as_uint(dist), __float_as_uint(r.W), __float_as_uint(r.invPdf));
+    nbr[2 * i + 1] = make_uint4(__float_as_uint(r.pos.x), __float_as_uint(r.pos.y), __float_as_uint(r.pos.z), r.light);
+}
+
 // GridCoordinatesFromHit + Uint3CubeEquals (cl/tools.cl:452-472) behind
 // isLightOccluded (cl/ris.cl:1-17).  A shadow ray that finds no opaque voxel
 // within its step budget counts as occluded.
@@ -348,6 +359,7 @@ __global__ __launch_bounds__(256) void k_initial(FrameArgs a)
         a.cur.seed[idx] = seed;
     }
     store_res(a.B, idx, res);
+    store_neighbour(a.nbr, idx, res, hitw, dist);
 }
 
 // ---------------------------------------------------------------------------
@@ -374,12 +386,19 @@ __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
             const int nx = x + f2i(cs * radius), ny = y + f2i(sn * radius);
             if ((nx == x && ny == y) || nx < 0 || ny < 0 || nx >= (int)a.W || ny >= (int)a.H) continue;
             const size_t ni = (size_t)ny * a.W + nx;
-            const uint32_t nm = a.B.m[ni];
-            if ((nm & 0xffffffu) == 0u) continue;
-            const uint32_t nh = a.cur.hitw[ni];
-            const float nd = a.cur.dist[ni];
-            if (!same_face(nh, hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist)) continue;
-            Res nr = load_res(a.B, ni);
+            // the neighbour's 32-byte record: one sector instead of five planes
+            const uint4 n0 = a.nbr[2 * ni];
+            if ((n0.x & 0xffffffu) == 0u) continue;
+            const float nd = __uint_as_float(n0.y);
+            if (!same_face((n0.x >> 3) & (7u << 24), hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist))
+                continue;
+            const uint4 n1 = a.nbr[2 * ni + 1];
+            Res nr;
+            nr.M = n0.x & 0xffffffu, nr.face = (n0.x >> 24) & 7u;
+            nr.W = __uint_as_float(n0.z), nr.invPdf = __uint_as_float(n0.w);
+            nr.pos = f3{__uint_as_float(n1.x), __uint_as_float(n1.y), __uint_as_float(n1.z)};
+            nr.light = n1.w;
+            nr.wsum = 0.0f, nr.pHat = 0.0f;  // both are recomputed by the reweigh below
             if (nr.light >= a.L.n) continue;
             const float4 la = a.L.a[nr.light], lb = a.L.b[nr.light];
             const float ph = phat(a.point, nr.pos, face_normal(nr.face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) *

@@ -135,29 +135,113 @@ __device__ __forceinline__ void softmax_stats(const MlpView& m, const uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
+// inference path (k_proposal, k_mlp_probs): one padded activation buffer per
+// wave (H1 is overwritten by H2, rows padded to 72 so the 16-byte operand
+// loads of neighbouring rows fall in different LDS banks) and the output
+// bias staged in LDS with -inf for clusters without lights, used as the MFMA
+// accumulator's initial value
+// ---------------------------------------------------------------------------
+constexpr int kLdX = 24;  // padded row strides (bf16 elements, 16-byte multiples)
+constexpr int kLdH = 72;
+
+struct __attribute__((aligned(16))) InferLds {
+    __bf16 X[64 * kLdX];  // inputs, sample-major
+    __bf16 H[64 * kLdH];  // relu(layer 1), then relu(layer 2)
+};
+
+// masked output bias: b3[c], or -inf for an empty or absent cluster
+__device__ __forceinline__ void stage_bias(const MlpView& m, float* sBias)
+{
+    for (int c = threadIdx.x; c < kMaxClusters; c += blockDim.x)
+        sBias[c] = (c < (int)m.nclusters && m.clusterStart[c + 1] > m.clusterStart[c]) ? m.b3[c] : -INFINITY;
+}
+
+__device__ void forward_infer(const MlpView& m, InferLds& s)
+{
+    for (int mb = 0; mb < 2; mb++)
+        for (int nb = 0; nb < 2; nb++) {
+            f32x16 acc = mfma_nt(s.X + mb * 32 * kLdX, kLdX, m.W1b + nb * 32 * 16, 16, 16, zero16());
+            const int j = nb * 32 + acc_col();
+            const float bj = m.b1[j];
+#pragma unroll
+            for (int q = 0; q < 16; q++) s.H[(mb * 32 + acc_row(q)) * kLdH + j] = to_bf16(fmaxf(acc[q] + bj, 0.0f));
+        }
+    __syncthreads();
+    // layer 2, one 32-sample row block at a time: its outputs read only its
+    // own rows of layer 1, so they can replace them in place
+    for (int mb = 0; mb < 2; mb++) {
+        const f32x16 h0 = mfma_nt(s.H + mb * 32 * kLdH, kLdH, m.W2b, 64, 64, zero16());
+        const f32x16 h1 = mfma_nt(s.H + mb * 32 * kLdH, kLdH, m.W2b + 32 * 64, 64, 64, zero16());
+        __syncthreads();
+        const int j = acc_col();
+        const float b0 = m.b2[j], b1 = m.b2[32 + j];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            s.H[(mb * 32 + acc_row(q)) * kLdH + j] = to_bf16(fmaxf(h0[q] + b0, 0.0f));
+            s.H[(mb * 32 + acc_row(q)) * kLdH + 32 + j] = to_bf16(fmaxf(h1[q] + b1, 0.0f));
+        }
+        __syncthreads();
+    }
+}
+
+// logits of cluster block mb for sample block nb (bias as initial value)
+__device__ __forceinline__ f32x16 logits_infer(const MlpView& m, const float* sBias, const InferLds& s, int mb, int nb)
+{
+    f32x16 acc;
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = sBias[mb * 32 + acc_row(q)];
+    return mfma_nt(m.W3b + mb * 32 * 64, 64, s.H + nb * 32 * kLdH, kLdH, 64, acc);
+}
+
+// softmax statistics of sample block nb (see softmax_stats)
+__device__ __forceinline__ void softmax_infer(const MlpView& m, const float* sBias, const InferLds& s, int nb, float& M,
+                                              float& S, float& Slo)
+{
+    float mx = -INFINITY, sum = 0.0f;
+#pragma unroll 1
+    for (int mb = 0; mb < 8; mb++) {
+        const f32x16 z = logits_infer(m, sBias, s, mb, nb);
+        float bm = mx;
+#pragma unroll
+        for (int q = 0; q < 16; q++) bm = fmaxf(bm, z[q]);
+        if (bm == -INFINITY) continue;  // nothing unmasked yet
+        float bs = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 16; q++) bs += __expf(z[q] - bm);  // exp(-inf) = 0
+        sum = (mx == -INFINITY ? 0.0f : sum * __expf(mx - bm)) + bs;
+        mx = bm;
+    }
+    const float omx = __shfl_xor(mx, 32), osum = __shfl_xor(sum, 32);
+    M = fmaxf(mx, omx);
+    const float mine = mx == -INFINITY ? 0.0f : sum * __expf(mx - M);
+    const float other = omx == -INFINITY ? 0.0f : osum * __expf(omx - M);
+    S = mine + other;
+    Slo = (threadIdx.x & 63) < 32 ? mine : other;  // mass held by the lower lane of the pair
+}
+
+// ---------------------------------------------------------------------------
 // k_proposal: per pixel, 32 stratified cluster draws from the defensive
 // mixture m(c | x) = (1 - eps) q(c | x) + eps P_c, where q is the network's
 // softmax over the non-empty clusters and P_c the cluster's share of the
 // total light power.  Candidate j's target is (j + u_j)/32 of the mixture
-// mass: one sorted walk of the CDF serves all 32, and the strata together
-// cover the distribution once, so the RIS weight p-hat/p stays unbiased.
-// Slot j receives (cluster, m_c); the candidate pass draws the light inside
-// the cluster in proportion to its power, so the source pdf of light i is
+// mass, u_j = hash(pixel stream, j): the strata together cover the
+// distribution once, so the RIS weight p-hat/p stays unbiased.  The lane
+// pair of a pixel splits the CDF (the lower lane holds the clusters with
+// (c & 4) == 0) and each lane walks its part once in cluster order, writing
+// slot j as (cluster, m_c); the candidate pass draws the light inside the
+// cluster in proportion to its power, so the source pdf of light i is
 // m_c * pmf_i / P_c (eps = 1 gives the plain power source exactly).
-// The walk only writes; the cluster table is staged in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32_t* __restrict__ candIdx,
                                                   float* __restrict__ candPdf, uint32_t salt)
 {
-    __shared__ ChunkLds lds[4];
-    __shared__ uint32_t sStart[kMaxClusters + 1];
+    __shared__ InferLds lds[4];
+    __shared__ float sBias[kMaxClusters];
     __shared__ float sPower[kMaxClusters];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    ChunkLds& s = lds[wv];
-    for (int c = threadIdx.x; c <= kMaxClusters; c += 256) {
-        sStart[c] = m.clusterStart[c];
-        if (c < kMaxClusters) sPower[c] = m.clusterPower[c];
-    }
+    InferLds& s = lds[wv];
+    stage_bias(m, sBias);
+    for (int c = threadIdx.x; c < kMaxClusters; c += 256) sPower[c] = m.clusterPower[c];
     // the wave's 64 pixels: an 8x8 tile (same map as the frame kernels)
     int tx, ty;
     tile_of_block(a.tileOrder, tx, ty);
@@ -184,11 +268,12 @@ __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32
             }
         }
 #pragma unroll
-        for (int i = 0; i < 16; i++) s.X[lane * 16 + i] = to_bf16(in[i]);
+        for (int i = 0; i < 16; i++) s.X[lane * kLdX + i] = to_bf16(in[i]);
     }
     __syncthreads();
-    forward_hidden(m, s);
+    forward_infer(m, s);
     const float keep = 1.0f - m.mix;
+    const size_t plane = (size_t)a.W * a.H;
     for (int nb = 0; nb < 2; nb++) {
         const int p = nb * 32 + (lane & 31);
         int x, y;
@@ -196,94 +281,90 @@ __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32
         const bool valid = x < (int)a.W && y < (int)a.H;
         const size_t idx = valid ? (size_t)y * a.W + x : 0;
         float M, S, Slo;
-        softmax_stats(m, sStart, s, nb, M, S, Slo);
-        // mixture mass held by the lower lane (its clusters are those with (c & 4) == 0)
+        softmax_infer(m, sBias, s, nb, M, S, Slo);
+        // mixture mass held by the lower lane
         const float massLo = keep * (Slo / S) + m.mix * m.powerLow;
-        // both lanes of the pair draw the same 64 uniforms from the pixel's
-        // proposal stream (the shading stream stays untouched)
-        uint32_t rs = wang_hash((valid ? a.cur.seed[idx] : 0u) ^ salt);
-        const bool upper = lane >= 32;
-        // walk: lower lane owns targets in [0, massLo), upper lane [massLo, 1)
-        int j = 0;
-        float t = 0.0f;
-        auto next_target = [&]() {
-            // advance to the next target this lane owns; returns false when done
-            while (j < 32) {
-                const float u = rnd(rs);
-                t = ((float)j + u) * (1.0f / 32.0f);
-                if ((t < massLo) != upper) {
-                    if (upper) t -= massLo;
-                    return true;
-                }
-                j++;
-            }
-            return false;
+        // target jitters from the pixel's proposal stream (the shading stream
+        // stays untouched); both lanes of the pair see the same u_j
+        const uint32_t ubase = wang_hash((valid ? a.cur.seed[idx] : 0u) ^ salt);
+        auto target = [&](int j) {
+            return ((float)j + (float)wang_hash(ubase + (uint32_t)j * 0x9e3779b9u) * 2.3283064365387e-10f) *
+                   (1.0f / 32.0f);
         };
-        bool more = next_target();
-        float cdf = 0.0f;
+        // slots [kBeg, kEnd) belong to this lane: the lower lane takes the
+        // targets below massLo
+        int kSplit = 0;
+        while (kSplit < 32 && target(kSplit) < massLo) kSplit++;
+        const bool upper = lane >= 32;
+        const int kBeg = upper ? kSplit : 0, kEnd = upper ? 32 : kSplit;
+        const float base = upper ? massLo : 0.0f;
+        int k = kBeg;
+        float tk = k < kEnd ? target(k) : 2.0f;  // next target of this lane
+        float cdf = base;
         uint32_t lastC = 0xffffffffu;
-        float lastP = 0.0f;
-        const size_t plane = (size_t)a.W * a.H;
+        float lastM = 0.0f;
+#pragma unroll 1
         for (int mb = 0; mb < 8; mb++) {
-            const f32x16 z = logits_block(m, sStart, s, mb, nb);
+            const f32x16 z = logits_infer(m, sBias, s, mb, nb);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const uint32_t c = (uint32_t)(mb * 32 + acc_row(q));
-                const float pq = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M);
-                const float mc = keep * (pq / S) + m.mix * sPower[c];
-                const float next = cdf + mc;
-                while (more && t < next && mc > 0.0f) {
-                    if (valid) {
-                        candIdx[(size_t)j * plane + idx] = c;
-                        candPdf[(size_t)j * plane + idx] = mc;
+                const float mc = keep * (__expf(z[q] - M) / S) + m.mix * sPower[c];
+                cdf = cdf + mc;
+                if (mc > 0.0f) {
+                    while (tk < cdf) {
+                        if (valid) {
+                            candIdx[(size_t)k * plane + idx] = c;
+                            candPdf[(size_t)k * plane + idx] = mc;
+                        }
+                        k++;
+                        tk = k < kEnd ? target(k) : 2.0f;
                     }
-                    j++;
-                    more = next_target();
+                    lastC = c, lastM = mc;
                 }
-                if (mc > 0.0f) lastC = c, lastP = mc;
-                cdf = next;
             }
         }
         // rounding can leave targets just above the walked mass: they take
         // the last cluster with mass; a lane without any mass marks them
         // empty (candidate skipped with weight 0)
-        while (more) {
+        for (; k < kEnd; k++)
             if (valid) {
-                candIdx[(size_t)j * plane + idx] = lastC;
-                candPdf[(size_t)j * plane + idx] = lastP;
+                candIdx[(size_t)k * plane + idx] = lastC;
+                candPdf[(size_t)k * plane + idx] = lastM;
             }
-            j++;
-            more = next_target();
-        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// probabilities for explicit inputs (test and evaluation hook)
+// probabilities for explicit inputs (test and evaluation hook), computed as
+// k_proposal computes them
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_mlp_probs(MlpView m, const float* __restrict__ x, uint32_t B,
                                                   float* __restrict__ out)
 {
-    __shared__ ChunkLds s;
+    __shared__ InferLds s;
+    __shared__ float sBias[kMaxClusters];
     const int lane = threadIdx.x & 63;
     const uint32_t base = blockIdx.x * 64;
+    stage_bias(m, sBias);
     {
         const uint32_t b = base + lane;
 #pragma unroll
-        for (int i = 0; i < 16; i++) s.X[lane * 16 + i] = to_bf16(b < B && i < 9 ? x[(size_t)b * 9 + i] : 0.0f);
+        for (int i = 0; i < 16; i++) s.X[lane * kLdX + i] = to_bf16(b < B && i < 9 ? x[(size_t)b * 9 + i] : 0.0f);
     }
     __syncthreads();
-    forward_hidden(m, s);
+    forward_infer(m, s);
     for (int nb = 0; nb < 2; nb++) {
         float M, S, Slo;
-        softmax_stats(m, m.clusterStart, s, nb, M, S, Slo);
+        softmax_infer(m, sBias, s, nb, M, S, Slo);
         const uint32_t b = base + nb * 32 + (lane & 31);
+#pragma unroll 1
         for (int mb = 0; mb < 8; mb++) {
-            const f32x16 z = logits_block(m, m.clusterStart, s, mb, nb);
+            const f32x16 z = logits_infer(m, sBias, s, mb, nb);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int c = mb * 32 + acc_row(q);
-                if (b < B) out[(size_t)b * 256 + c] = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M) / S;
+                if (b < B) out[(size_t)b * 256 + c] = __expf(z[q] - M) / S;
             }
         }
     }

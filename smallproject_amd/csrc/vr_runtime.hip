@@ -53,6 +53,7 @@ struct vr_ctx {
     float4* rb[2] = {};
     uint32_t* rm[2] = {};
     float4* frame = nullptr;
+    uint4* nbr = nullptr;     // [2 per pixel] neighbour records of the B reservoirs
     int cur = 1;              // current hit buffer (swapped before each frame)
     bool committed = false;
     bool framed = false;
@@ -77,7 +78,7 @@ struct vr_ctx {
     float* lightShare = nullptr;        // [lights] pmf_i / P_c
     float* clusterPower = nullptr;      // [kMaxClusters] P_c
     float powerLow = 0.0f;              // sum of P_c over clusters with (c & 4) == 0
-    float mix = 0.1f;                   // defensive mixture weight of the power source
+    float mix = 0.25f;                  // defensive mixture weight of the power source
     std::vector<vr_light> hostLights;   // the uploaded light list
     uint32_t nclusters = 0, clusterLightCap = 0, clusteredLights = 0;
     uint32_t* candIdx = nullptr;  // [kCandidates][W*H]
@@ -281,6 +282,7 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
     a.A = ResPlanes{c->ra[0], c->rb[0], c->rm[0]};
     a.B = ResPlanes{c->ra[1], c->rb[1], c->rm[1]};
     a.frame = c->frame;
+    a.nbr = c->nbr;
     a.emitLut = c->emitLut;
     a.W = c->cfg.width, a.H = c->cfg.height;
     a.point = c->cfg.light_mode == VR_LIGHTS_POINT ? 1u : 0u;
@@ -421,7 +423,7 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 32768)) &&
            ok(dalloc(&c->bricks, (size_t)cfg->brick_capacity * VR_BRICKSIZE)) &&
            ok(dalloc(&c->stageIds, VR_MAXCOMMITS)) && ok(dalloc(&c->stageData, (size_t)VR_MAXCOMMITS * VR_BRICKSIZE)) &&
-           ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n));
+           ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n)) && ok(dalloc(&c->nbr, 2 * n));
     for (int i = 0; good && i < 2; i++)
         good = ok(dalloc(&c->hitw[i], n)) && ok(dalloc(&c->dist[i], n)) && ok(dalloc(&c->seed[i], n)) &&
                ok(dalloc(&c->ra[i], n)) && ok(dalloc(&c->rb[i], n)) && ok(dalloc(&c->rm[i], n));
@@ -451,6 +453,7 @@ int vr_destroy(vr_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->grid), dfree(c->uber), dfree(c->bricks), dfree(c->stageIds), dfree(c->stageData);
     dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf), dfree(c->emitLut), dfree(c->frame);
+    dfree(c->nbr);
     for (int i = 0; i < 2; i++) {
         dfree(c->hitw[i]), dfree(c->dist[i]), dfree(c->seed[i]);
         dfree(c->ra[i]), dfree(c->rb[i]), dfree(c->rm[i]);

@@ -17,11 +17,11 @@ Equal time.  Both methods render the same world, camera and frame count
 from reset temporal history; per-frame GPU time comes from the stage
 events (the learned method's proposal kernel is part of its candidate
 stage).  Before the comparison the network is trained online along the
-configuration's camera path, one training step per frame from the plain
-method's final reservoirs (2^16 samples, Adam lr 1e-3); during the
-comparison its weights are frozen.  Reported: the plain method's MSE after
-8 frames, the number of learned frames (and their time) needed to reach it,
-and the learned MSE at the plain method's 8-frame time.
+configuration's camera path from the plain method's final reservoirs (2^16
+samples per frame in 16 minibatches, Adam lr 1e-3); during the comparison
+its weights are frozen.  Reported: the plain method's MSE after 8 frames,
+the number of learned frames (and their time) needed to reach it, and the
+learned MSE at the plain method's 8-frame time.
 """
 from __future__ import annotations
 
@@ -107,8 +107,8 @@ def plain_mse(name: str, width=None, height=None, frames: int = 8, gt_spp: int =
 
 
 def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_frames=None, eval_frames: int = 16,
-               gt_spp: int = 4096, nsamples: int = 1 << 16, lr: float = 1e-3, train_steps: int = 1,
-               nclusters: int = 256, eval_frame=None) -> dict:
+               gt_spp: int = 4096, nsamples: int = 1 << 16, lr: float = 1e-3, train_steps: int = 16,
+               nclusters: int = 256, eval_frame=None, mix: float = 0.25) -> dict:
     base = CONFIGS[name]
     scP = dataclasses.replace(base, source=_lib.SOURCE_POWER, quirks=EVAL_QUIRKS)
     scL = dataclasses.replace(base, source=_lib.SOURCE_LEARNED, quirks=EVAL_QUIRKS)
@@ -121,6 +121,7 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
     rL = wL.renderer
     clusters = wL.cluster_lights(min(nclusters, len(wL.lights())), 8, 1)
     rL.set_light_clusters(clusters, int(clusters.max()) + 1)
+    rL.set_proposal_mix(mix)
     # online training along the camera path from the plain method's frames
     T = base.frames if train_frames is None else train_frames
     losses = []
@@ -146,7 +147,7 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
     k_eq = max(1, min(eval_frames, int(k_ref * msP / msL)))
     out = {
         "config": name, "resolution": [W, H], "camera_frame": f, "gt_spp": gt_spp, "gt_noise_mse": noise,
-        "clusters": int(clusters.max()) + 1, "train_frames": T, "train_steps_per_frame": train_steps,
+        "clusters": int(clusters.max()) + 1, "mix": mix, "train_frames": T, "train_steps_per_frame": train_steps,
         "train_samples": nsamples, "lr": lr, "train_loss_first": losses[0], "train_loss_last": losses[-1],
         "plain": {"ms_per_frame": msP, "stage_ms": stP, "mse_per_frame": eP},
         "learned": {"ms_per_frame": msL, "stage_ms": stL, "mse_per_frame": eL},
@@ -168,7 +169,8 @@ def main(argv=None) -> int:
     ap.add_argument("--height", type=int)
     ap.add_argument("--gt-spp", type=int, default=4096)
     ap.add_argument("--train-frames", type=int)
-    ap.add_argument("--train-steps", type=int, default=1)
+    ap.add_argument("--train-steps", type=int, default=16)
+    ap.add_argument("--mix", type=float, default=0.25, help="eps of the proposal's defensive power mixture")
     ap.add_argument("--eval-frames", type=int, default=16)
     ap.add_argument("--json")
     args = ap.parse_args(argv)
@@ -177,7 +179,7 @@ def main(argv=None) -> int:
         t0 = time.perf_counter()
         if CONFIGS[name].clusters:
             res = equal_time(name, args.width, args.height, args.train_frames, args.eval_frames, args.gt_spp,
-                             train_steps=args.train_steps)
+                             train_steps=args.train_steps, mix=args.mix)
         else:
             res = plain_mse(name, args.width, args.height, 8, args.gt_spp)
         res["seconds"] = round(time.perf_counter() - t0, 1)
