@@ -90,6 +90,14 @@ int vrw_load_vx(vrw_world* w, const char* path, int x, int y, int z, uint32_t* s
 /* CreateSprite over a box + SaveSprite (world.cpp:1307-1325, 1347-1386),
  * 32-bit payload */
 int vrw_save_vx(const vrw_world* w, const char* path, int x, int y, int z, int sx, int sy, int sz);
+/* the world editor's world file (WorldEditor::LoadWorld, worldeditor.cpp:
+ * 1347-1452): clear, write the listed top-grid cells and bricks, find the
+ * lights, optimize bricks (the device copy follows on vrw_commit or
+ * vrw_force_sync).  stats (optional, 3 u32): grid entries, bricks, lights */
+int vrw_load_nbt(vrw_world* w, const char* path, uint32_t* stats);
+/* WorldEditor::SaveWorld (worldeditor.cpp:1454-1600): non-empty cells in
+ * index order, brick ids renumbered from 0 */
+int vrw_save_nbt(const vrw_world* w, const char* path);
 
 /* ---- image metrics and raster dumps (mse.cpp:33-65, mygame.cpp:518-549) ---- */
 /* out[0] = MSE, out[1] = mean absolute error over the rgb channels of two

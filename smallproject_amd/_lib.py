@@ -123,6 +123,8 @@ SIGNATURES = {
     "vrw_stage_times": (C.c_int, [_P, _P]),
     "vrw_load_vx": (C.c_int, [_P, C.c_char_p, C.c_int, C.c_int, C.c_int, _P]),
     "vrw_save_vx": (C.c_int, [_P, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "vrw_load_nbt": (C.c_int, [_P, C.c_char_p, _P]),
+    "vrw_save_nbt": (C.c_int, [_P, C.c_char_p]),
     "vrw_image_error": (C.c_int, [_P, _P, _u32, _u32, _P]),
     "vrw_write_dat": (C.c_int, [C.c_char_p, _u32, _u32, _P]),
     "vrw_read_dat": (C.c_int, [C.c_char_p, _P, _P, _P, _sz]),

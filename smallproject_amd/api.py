@@ -364,6 +364,15 @@ class World:
         self._chk(lib().vrw_save_vx(self.h, str(path).encode(), x, y, z, sx, sy, sz), f"save_vx {path}")
 
     # scenes
+    def load_nbt(self, path) -> dict:
+        """The world editor's world file (worlddata.nbt)."""
+        st = np.zeros(3, np.uint32)
+        self._chk(lib().vrw_load_nbt(self.h, str(path).encode(), ptr(st)), "load_nbt")
+        return {"grid_entries": int(st[0]), "bricks": int(st[1]), "lights": int(st[2])}
+
+    def save_nbt(self, path):
+        self._chk(lib().vrw_save_nbt(self.h, str(path).encode()), "save_nbt")
+
     def scene_terrain(self, size: int, scale: int, seed: int):
         self._chk(lib().vrw_scene_terrain(self.h, size, scale, seed), "scene_terrain")
 

@@ -21,7 +21,7 @@ LIBDIR = PKG / "lib"
 LIBNAME = "libvoxelrestir.so"
 
 HIP_SOURCES = ["vr_kernels.hip", "vr_mlp.hip", "vr_runtime.hip"]
-CXX_SOURCES = ["world.cpp", "world_capi.cpp", "scenes.cpp", "vxio.cpp", "imageio.cpp"]
+CXX_SOURCES = ["world.cpp", "world_capi.cpp", "scenes.cpp", "vxio.cpp", "nbtio.cpp", "imageio.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-ffp-contract=off", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}"]
 # Packed FP32 (v_pk_*_f32) instructions are disabled for all device code.

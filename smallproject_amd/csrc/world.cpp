@@ -117,6 +117,25 @@ uint32_t World::NewBrick()
     return id;
 }
 
+bool World::SetGridCell(uint32_t index, uint32_t value)
+{
+    if (index >= VR_GRIDSIZE) return false;
+    grid[index] = value;
+    gridDirty = true;
+    return true;
+}
+
+bool World::LoadBrick(uint32_t offset, const uint32_t* payload, uint32_t zeroCount)
+{
+    if (offset >= cap) return false;
+    std::memcpy(bricks + (size_t)offset * VR_BRICKSIZE, payload, VR_BRICKSIZE * 4);
+    zeroes[offset] = zeroCount;
+    Mark(offset);
+    (void)NewBrick();  // keeps the free ring in step with the loaded bricks
+    if (offset + 1 > highWater) highWater = offset + 1;
+    return true;
+}
+
 void World::FreeBrick(uint32_t idx)
 {
     // FreeBrick, template/world.h:596-606

@@ -109,6 +109,13 @@ public:
     uint32_t BrickCapacity() const { return cap; }
     uint32_t DirtyBricks(std::vector<uint32_t>* ids) const;
 
+    // raw state writes of WorldEditor::LoadWorld (worldeditor.cpp:1376-1426):
+    // a top-grid cell, and a brick's payload and zero count at a pool
+    // offset (marked dirty, with one ring allocation per loaded brick as the
+    // reference makes); false if the index or offset is out of range
+    bool SetGridCell(uint32_t index, uint32_t value);
+    bool LoadBrick(uint32_t offset, const uint32_t* payload, uint32_t zeroCount);
+
     // extensions beyond the reference
     void SetFocalDistance(float f) { focal = f; }  // 2.2 in world.cpp:2155-2157
     float GetFocalDistance() const { return focal; }

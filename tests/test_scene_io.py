@@ -87,3 +87,113 @@ def test_malformed_files_are_rejected(tmp_path):
         w.load_vx(trunc)
     with pytest.raises(_lib.VrError):
         w.load_vx(tmp_path / "missing.vx")
+
+
+# ---- the world editor's world file (worldeditor.cpp:13-172, 1347-1600) ----
+# Fixture: tests/golden/worlddata.nbt is the reference's own saved editor world.
+
+def _parse_nbt(data: bytes):
+    """Independent reader of the little-endian NBT variant: returns
+    {grid index: value} and {brick offset: (payload u32[512], zeroes)}."""
+    pos = 0
+
+    def take(n):
+        nonlocal pos
+        b = data[pos:pos + n]
+        assert len(b) == n
+        pos += n
+        return b
+
+    def tag(named):
+        t = take(1)[0]
+        if t == 0:
+            return (0, None, None)
+        name = take(struct.unpack("<h", take(2))[0]).decode() if named else None
+        if t == 3:
+            return (t, name, struct.unpack("<I", take(4))[0])
+        if t == 7:
+            return (t, name, take(struct.unpack("<i", take(4))[0]))
+        if t == 9:
+            take(1)
+            return (t, name, [tag(False) for _ in range(struct.unpack("<i", take(4))[0])])
+        if t == 10:
+            kids = []
+            while True:
+                kids.append(tag(True))
+                if kids[-1][0] == 0:
+                    return (t, name, kids)
+        raise AssertionError(f"tag type {t}")
+
+    grid, bricks = {}, {}
+    while pos < len(data):
+        t = tag(True)
+        for comp in t[2]:
+            d = {k[1]: k[2] for k in comp[2] if k[0]}
+            if t[1] == "grid list":
+                grid[d["grid index"]] = d["grid value"]
+            else:
+                bricks[d["brick buffer offset"]] = (np.frombuffer(d["brick value"], np.uint32), d["brick zeroes"])
+    return grid, bricks
+
+
+def _cell_voxels(index):
+    bx, bz, by = index % 128, (index // 128) % 128, index // 16384
+    lx, ly, lz = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    return bx * 8 + lx.ravel(), by * 8 + ly.ravel(), bz * 8 + lz.ravel(), lx.ravel() + 8 * ly.ravel() + 64 * lz.ravel()
+
+
+def _expected_voxels(grid, bricks):
+    """(x, y, z, value) of every voxel the file defines."""
+    out = []
+    for index, v in grid.items():
+        xs, ys, zs, li = _cell_voxels(index)
+        vals = bricks[v >> 1][0][li] if v & 1 else np.full(512, v >> 1, np.uint32)
+        out.append((xs, ys, zs, vals))
+    return [np.concatenate(c) for c in zip(*out)]
+
+
+def test_editor_world_file_loads_every_voxel_and_light():
+    data = (GOLDEN / "worlddata.nbt").read_bytes()
+    grid, bricks = _parse_nbt(data)
+    assert len(grid) == 41 and len(bricks) == 37
+    for payload, zeroes in bricks.values():  # the stored zero counts are consistent
+        assert zeroes == int(np.sum(payload == 0))
+    w = World(default_config(16, 16))
+    st = w.load_nbt(GOLDEN / "worlddata.nbt")
+    assert st["grid_entries"] == 41 and st["bricks"] == 37
+    xs, ys, zs, vals = _expected_voxels(grid, bricks)
+    got = np.array([w.get(int(x), int(y), int(z)) for x, y, z in zip(xs, ys, zs)], np.uint32)
+    assert np.array_equal(got, vals)
+    # FindLightsInWord (world.cpp:248-284): one size-8 light per emissive solid
+    # cell, one size-1 light per emissive voxel of a brick
+    emissive_cells = sum(1 for v in grid.values() if not v & 1 and (v >> 17) & 255)
+    emissive_voxels = sum(int(np.sum((bricks[v >> 1][0] >> 16) & 255 != 0)) for v in grid.values() if v & 1)
+    assert st["lights"] == emissive_cells + emissive_voxels == len(w.lights())
+    assert st["lights"] >= 1
+
+
+def test_editor_world_file_roundtrip(tmp_path):
+    w = World(default_config(16, 16))
+    w.load_nbt(GOLDEN / "worlddata.nbt")
+    out = tmp_path / "saved.nbt"
+    w.save_nbt(out)
+    grid2, bricks2 = _parse_nbt(out.read_bytes())
+    # renumbered bricks are 0..n-1 and every cell's voxels are unchanged
+    assert sorted(bricks2) == list(range(len(bricks2)))
+    w2 = World(default_config(16, 16))
+    w2.load_nbt(out)
+    xs, ys, zs, vals = _expected_voxels(grid2, bricks2)
+    got = np.array([w2.get(int(x), int(y), int(z)) for x, y, z in zip(xs, ys, zs)], np.uint32)
+    assert np.array_equal(got, vals)
+    g1 = w.grid().copy()
+    assert np.array_equal(g1 != 0, w2.grid() != 0)
+
+
+def test_malformed_editor_world_files_are_rejected(tmp_path):
+    data = (GOLDEN / "worlddata.nbt").read_bytes()
+    w = World(default_config(16, 16))
+    for bad in (data[:100], data[:-7], b"\x09\x05\x00grid", bytes([10]) * 64):
+        p = tmp_path / "bad.nbt"
+        p.write_bytes(bad)
+        with pytest.raises(_lib.VrError):
+            w.load_nbt(p)
