@@ -54,7 +54,9 @@ def cpu_baseline(sc, nthreads: int) -> dict:
     w = configs.build_world(sc)
     view = O.View(w.grid(), w.bricks())
     lut = np.array(list(w.cfg.emit_lut), np.float32)
-    ocfg = O.make_config(sc.width, sc.height, sc.point, gfxexp=1, quirks=sc.quirks, source=sc.source, lut=lut)
+    from smallproject_amd import _lib
+    ocfg = O.make_config(sc.width, sc.height, sc.point, gfxexp=1, quirks=sc.quirks, source=sc.source, lut=lut,
+                         spatial_mis=bool(sc.flags & _lib.FLAG_MIS_SPATIAL))
     fr = O.Frames(ocfg, view, w.lights())
     t = 0.0
     for f in range(2):
@@ -138,6 +140,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--tile-order", default="auto", choices=["auto", "linear", "xcd_rows", "xcd_cols"],
                     help="workgroup->tile order; auto times each on the first frames after warm-up")
+    ap.add_argument("--spatial", default="biased", choices=["biased", "mis"],
+                    help="spatial merge: the reference's (biased) or MIS-weighted with a visibility ray per neighbour")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only the known-size copy used to scale HBM byte counters")
     args = ap.parse_args()
@@ -148,8 +152,12 @@ def main() -> int:
     if args.calibrate:
         return calibrate()
 
-    from smallproject_amd import configs
+    import dataclasses
+
+    from smallproject_amd import _lib, configs
     sc = configs.CONFIGS[args.config]
+    if args.spatial == "mis":
+        sc = dataclasses.replace(sc, flags=sc.flags | _lib.FLAG_MIS_SPATIAL)
     t0 = time.perf_counter()
     w = configs.build_world(sc)
     log(f"bench: built {sc.name} host world in {time.perf_counter() - t0:.1f} s "
@@ -206,7 +214,9 @@ def main() -> int:
         "data": "synthetic (seeded procedural world, lights and camera path)",
         "config": {"workload": f"{sc.name}: plain Voxel ReSTIR, area lights, {sc.width}x{sc.height}, "
                                f"1024^3 world, {len(w.lights())} emitters, M={sc.candidates}, "
-                               f"{sc.taps} taps r={sc.radius}, temporal clamp {sc.temporal_clamp}x",
+                               f"{sc.taps} taps r={sc.radius}, temporal clamp {sc.temporal_clamp}x"
+                               + (", MIS spatial merge" if args.spatial == "mis" else ""),
+                   "spatial_merge": args.spatial,
                    "resolution": [sc.width, sc.height], "frames_in_path": sc.frames,
                    "tile_order": tile_order,
                    "tile_order_stage_ms": None if tune_ms is None else

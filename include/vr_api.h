@@ -96,6 +96,18 @@ typedef struct vr_render_params {
 
 /* compile-time switches of the reference (template/common.h) as a
  * per-context configuration */
+/* vr_config.flags: spatial reuse merges with generalized balance-heuristic
+ * MIS weights instead of the reference's 1/M normalisation (unbiased; no
+ * counterpart in the reference).  The merged sample and the random stream
+ * are the reference's; only W changes, to
+ *     W = wsum * T_sel(y) / (pHat * sum_s M_s T_s(y))
+ * over the own reservoir and every combined neighbour s, T_s(y) being the
+ * target of s's pixel times its visibility of y (one shadow ray per
+ * combined neighbour; the own term has no ray).  At most VR_MIS_MAX_TAPS
+ * spatial taps. */
+#define VR_FLAG_MIS_SPATIAL (1u << 8)
+#define VR_MIS_MAX_TAPS 32
+
 typedef struct vr_config {
     uint32_t width, height;   /* SCRWIDTH, SCRHEIGHT */
     uint32_t light_mode;      /* VR_LIGHTS_* */
@@ -103,7 +115,8 @@ typedef struct vr_config {
     uint32_t quirks;          /* VR_QUIRK_* */
     uint32_t source_pdf;      /* VR_SOURCE_* */
     uint32_t brick_capacity;  /* device brick pool size in bricks (reference: VR_BRICKCOUNT) */
-    uint32_t flags;           /* bits 2s..2s+1: tile order VR_TILES_* of stage s; bits 8+ zero */
+    uint32_t flags;           /* bits 2s..2s+1: tile order VR_TILES_* of stage s;
+                                 VR_FLAG_MIS_SPATIAL; other bits zero */
     float emit_lut[256];      /* EmitStrength(v) = emit_lut[(v >> 16) & 255]; the
                                  reference's is emit_lut[m] = m (common.h:208-211) */
 } vr_config;

@@ -43,7 +43,7 @@ class Config(C.Structure):
                 ("gfxexp", C.c_uint32), ("quirks", C.c_uint32), ("source_pdf", C.c_uint32),
                 ("emit_lut", C.c_float * 256), ("cand_idx", C.c_void_p), ("cand_pdf", C.c_void_p),
                 ("cl_start", C.c_void_p), ("cl_lights", C.c_void_p), ("cl_prob", C.c_void_p),
-                ("cl_alias", C.c_void_p), ("cl_share", C.c_void_p)]
+                ("cl_alias", C.c_void_p), ("cl_share", C.c_void_p), ("spatial_mis", C.c_uint32)]
 
 
 class WorldView(C.Structure):
@@ -134,9 +134,10 @@ def ptr(a: np.ndarray) -> int:
 
 
 def make_config(width: int, height: int, point: bool, gfxexp: int = 1, quirks: int = QUIRKS_REFERENCE,
-                source: int = SOURCE_UNIFORM, lut=None) -> Config:
+                source: int = SOURCE_UNIFORM, lut=None, spatial_mis: bool = False) -> Config:
     c = Config()
     c.width, c.height, c.point_lights, c.gfxexp, c.quirks, c.source_pdf = width, height, int(point), gfxexp, quirks, source
+    c.spatial_mis = int(spatial_mis)
     lut = np.arange(256, dtype=np.float32) if lut is None else np.asarray(lut, dtype=np.float32)
     for i in range(256):
         c.emit_lut[i] = float(lut[i])

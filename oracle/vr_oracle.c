@@ -750,9 +750,59 @@ void or_stage_initial(const or_config* c, const or_params* p, const or_world_vie
         }
 }
 
+/* Unbiased contribution weight of the spatially merged sample y under
+ * generalized balance-heuristic MIS (Bitterli et al. 2020, "Spatiotemporal
+ * reservoir resampling", sec. 4: weights evaluated for the selected sample
+ * only).  Input s of the merge (own reservoir, then each combined neighbour
+ * in tap order) has target T_s(y) = phat_s(y) * V_s(y): the target of its
+ * pixel times visibility from that pixel (one shadow ray per neighbour);
+ * the own term omits the ray because the shading pass multiplies by the
+ * own visibility.  With resampling weights phat(y_s) W_s M_s the estimate
+ * is unbiased when
+ *     W = wsum * T_sel(y) / (pHat * sum_s M_s T_s(y)),
+ * pHat being the merged reservoir's own target of y (phat * invPdf; the
+ * common invPdf cancels between T_sel and the sum).  W = 0 when the
+ * denominator is 0 or not finite. */
+static float mis_weight(const or_config* c, const or_params* p, const or_world_view* w, const or_light* lights,
+                        const or_ray* hits, const or_reservoir* in, const or_reservoir* res, const float sp[3],
+                        const float N[3], const float brdf[3], uint32_t M0, const size_t* acc, int nacc, int sel)
+{
+    if (!(res->wsum > 0.0f) || res->lightIndex >= p->numberOfLights) return 0.0f;
+    const or_light* light = &lights[res->lightIndex];
+    const float pos[3] = {res->pos.x, res->pos.y, res->pos.z};
+    const float nl[3] = {res->Nlight.x, res->Nlight.y, res->Nlight.z};
+    const float p0 = or_eval_phat(c, pos, nl, sp, N, brdf, light->voxel, light->size);
+    float den = (float)M0 * p0;
+    float psel = sel < 0 ? p0 : 0.0f;
+    for (int j = 0; j < nacc; j++) {
+        const or_ray* nr = &hits[acc[j]];
+        const float nD[3] = {nr->D.x, nr->D.y, nr->D.z};
+        float nN[3], nbrdf[3], nsp[3], nspo[3];
+        voxel_normal(nr->side, nD, nN);
+        or_to_float_rgb(nr->voxel, nbrdf);
+        for (int k = 0; k < 3; k++) {
+            nbrdf[k] = nbrdf[k] * INVPI;
+            nsp[k] = nD[k] * nr->dist + p->E[k];
+            nspo[k] = nsp[k] + 0.1f * nN[k];
+        }
+        float pj = or_eval_phat(c, pos, nl, nsp, nN, nbrdf, light->voxel, light->size);
+        if (pj > 0.0f) {
+            float R[3], L[3];
+            R[0] = pos[0] - nspo[0], R[1] = pos[1] - nspo[1], R[2] = pos[2] - nspo[2];
+            normalize3(R, L);
+            if (light_occluded(c, w, light, nspo, L)) pj = 0.0f;
+        }
+        den += (float)in[acc[j]].M * pj;
+        if (j == sel) psel = pj;
+    }
+    const float d = res->pHat * den;
+    const float W = (res->wsum * psel) / d;
+    return (d > 0.0f && isfinite(W)) ? W : 0.0f;
+}
+
 /* perPixelSpatialResampling + _perPixelSpatialResampling,
  * cl/ris.cl:126-151, cl/spatialSampling.cl:2-67 */
-void or_stage_spatial(const or_config* c, const or_params* p, const or_light* lights,
+void or_stage_spatial(const or_config* c, const or_params* p, const or_world_view* w, const or_light* lights,
                       or_ray* hits, or_reservoir* out, const or_reservoir* in, int y0, int y1)
 {
     const uint32_t W = c->width, H = c->height;
@@ -769,6 +819,11 @@ void or_stage_spatial(const or_config* c, const or_params* p, const or_light* li
                 voxel_normal(ray->side, D, N);
                 or_to_float_rgb(ray->voxel, brdf);
                 for (int k = 0; k < 3; k++) brdf[k] = brdf[k] * INVPI, sp[k] = D[k] * dist + p->E[k];
+                /* MIS-weighted merge: the combined neighbours (pixel index)
+                 * and which input the selected sample came from (-1 = own) */
+                const uint32_t M0 = res.M;
+                size_t acc[OR_MIS_MAX_TAPS];
+                int nacc = 0, sel = -1;
                 for (int i = 0; i < (int)p->spatialTaps; i++) {
                     const float angle = or_random_float(seed) * TWOPI;
                     const float radius = (float)(p->spatialRadius - 1u) * sqrtf(or_random_float(seed)) + 1.0f;
@@ -788,11 +843,19 @@ void or_stage_spatial(const or_config* c, const or_params* p, const or_light* li
                             const float nnl[3] = {nr.Nlight.x, nr.Nlight.y, nr.Nlight.z};
                             const float ph = or_eval_phat(c, npos, nnl, sp, N, brdf, nl->voxel, nl->size) * nr.invPdf;
                             or_reservoir_reweigh(&nr, ph, nr.M);
-                            or_reservoir_combine(&res, &nr, or_random_float(seed));
+                            const float r0 = or_random_float(seed);
+                            /* the test of UpdateReservoir: does this input take over? */
+                            const int taken = r0 < nr.wsum / fmaxf(1e-5f, res.wsum + nr.wsum);
+                            or_reservoir_combine(&res, &nr, r0);
+                            if (c->spatial_mis && nacc < OR_MIS_MAX_TAPS) {
+                                if (taken) sel = nacc;
+                                acc[nacc++] = ni;
+                            }
                         }
                     }
                 }
                 or_reservoir_adjust(&res, res.pHat);
+                if (c->spatial_mis) res.W = mis_weight(c, p, w, lights, hits, in, &res, sp, N, brdf, M0, acc, nacc, sel);
             }
             out[idx] = res;
         }
@@ -915,7 +978,7 @@ void or_render_frame(or_state* s, const or_config* c, const or_params* p,
         or_stage_initial(c, p, w, lights, src, prev, cur, s->res[iOut], s->res[iIn], y, y + rows < H ? y + rows : H);
     memcpy(s->initial, s->res[iOut], (size_t)s->W * s->H * sizeof(or_reservoir));
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-    for (int y = 0; y < H; y += rows) or_stage_spatial(c, p, lights, cur, s->res[sOut], s->res[sIn], y, y + rows < H ? y + rows : H);
+    for (int y = 0; y < H; y += rows) or_stage_spatial(c, p, w, lights, cur, s->res[sOut], s->res[sIn], y, y + rows < H ? y + rows : H);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (int y = 0; y < H; y += rows)
         or_stage_shade(c, p, w, lights, cur, s->frame, s->res[hOut], s->res[hIn], y, y + rows < H ? y + rows : H);

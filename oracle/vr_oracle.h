@@ -116,7 +116,12 @@ typedef struct or_config {
     const float* cl_prob;       /* alias tables aligned with cl_lights */
     const uint32_t* cl_alias;   /* cluster-local indices */
     const float* cl_share;      /* by light id */
+    /* spatial merge with generalized balance-heuristic MIS weights and one
+     * visibility ray per combined neighbour (unbiased); at most
+     * OR_MIS_MAX_TAPS combined neighbours enter the weights */
+    uint32_t spatial_mis;
 } or_config;
+#define OR_MIS_MAX_TAPS 32
 
 /* read-only view of a device world: top grid, brick pool, uber grid */
 typedef struct or_world_view {
@@ -180,7 +185,7 @@ void or_stage_initial(const or_config* c, const or_params* p, const or_world_vie
                       const or_light* lights, const or_alias* src, const or_ray* prevHits,
                       or_ray* hits, or_reservoir* out, const or_reservoir* prevRes,
                       int y0, int y1);
-void or_stage_spatial(const or_config* c, const or_params* p, const or_light* lights,
+void or_stage_spatial(const or_config* c, const or_params* p, const or_world_view* w, const or_light* lights,
                       or_ray* hits, or_reservoir* out, const or_reservoir* in, int y0, int y1);
 void or_stage_shade(const or_config* c, const or_params* p, const or_world_view* w,
                     const or_light* lights, const or_ray* hits, float* frame,

@@ -23,6 +23,7 @@ QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
 QUIRKS_REFERENCE = 7
 TRACE_ALL, TRACE_OPAQUE = 0, 1
 TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS = 0, 1, 2
+FLAG_MIS_SPATIAL, MIS_MAX_TAPS = 1 << 8, 32
 NUM_STAGES = 4
 STAGE_NAMES = ("primary", "initial", "spatial", "shade")
 GRIDSIZE = 128 ** 3

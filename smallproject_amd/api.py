@@ -18,11 +18,12 @@ from ._lib import (LIGHT_DTYPE, VrConfig, VrLight, VrRenderParams, check, lib, p
 
 def default_config(width: int, height: int, *, point: bool = False, quirks: int = _lib.QUIRKS_REFERENCE,
                    gfxexp: int = 1, source: int = _lib.SOURCE_UNIFORM, capacity: int = _lib.BRICKCOUNT,
-                   lut=None) -> VrConfig:
+                   lut=None, flags: int = 0) -> VrConfig:
     cfg = VrConfig()
     lib().vr_default_config(C.byref(cfg), width, height)
     cfg.light_mode = _lib.LIGHTS_POINT if point else _lib.LIGHTS_AREA
     cfg.quirks, cfg.gfxexp, cfg.source_pdf, cfg.brick_capacity = quirks, gfxexp, source, capacity
+    cfg.flags = flags
     if lut is not None:
         lut = np.asarray(lut, dtype=np.float32)
         for i in range(256):

@@ -55,12 +55,13 @@ class SceneConfig:
     cluster_y: tuple = (16, 128)
     quirks: int = _lib.QUIRKS_REFERENCE
     source: int = _lib.SOURCE_UNIFORM
+    flags: int = 0                # vr_config.flags (VR_FLAG_MIS_SPATIAL)
     extra: dict = field(default_factory=dict)
 
     def config(self, width: int | None = None, height: int | None = None):
         lut = None if self.lut is None else log_lut(*self.lut)
         return default_config(width or self.width, height or self.height, point=self.point, quirks=self.quirks,
-                              source=self.source, lut=lut)
+                              source=self.source, lut=lut, flags=self.flags)
 
     def camera(self, frame: int):
         return camera_path(self.camera_kind, self.camera_seed, frame, self.frames, self.world_size, self.scale,

@@ -36,6 +36,7 @@ struct FrameArgs {
     uint32_t point, gfxexp, source, axisOnlyFace, doubleRadiance;
     int32_t shadowSteps;
     uint32_t tileOrder;  // VR_TILES_*
+    uint32_t misSpatial; // spatial merge with MIS weights (VR_FLAG_MIS_SPATIAL)
     // learned source: candidate slot j of pixel i at [j * W*H + i] holds a
     // cluster and its mixture mass; inside the cluster the light is drawn
     // with the cluster's power alias table

@@ -365,6 +365,63 @@ __global__ __launch_bounds__(256) void k_initial(FrameArgs a)
 // ---------------------------------------------------------------------------
 // k_spatial: spatial reuse over a disc (cl/spatialSampling.cl:2-67)
 // ---------------------------------------------------------------------------
+
+// spatial tap i of a pixel: offset on the disc (cl/spatialSampling.cl:22-27)
+__device__ __forceinline__ void tap_offset(uint32_t& seed, float rad, int& dx, int& dy)
+{
+    const float angle = rnd(seed) * kTwoPi;
+    const float radius = rad * sqrtf(rnd(seed)) + 1.0f;
+    float sn, cs;
+    sincos_det(angle, sn, cs);
+    dx = f2i(cs * radius), dy = f2i(sn * radius);
+}
+
+// Unbiased contribution weight of the merged sample under generalized
+// balance-heuristic MIS (weights evaluated for the selected sample only):
+//     W = wsum * T_sel(y) / (pHat * sum_s M_s T_s(y))
+// over the own reservoir (s = own, M0, T = phat without a ray: shading
+// multiplies by the own visibility) and the combined neighbours in tap
+// order (T = phat at the neighbour's surface point times its visibility of
+// y, one shadow ray each).  The neighbours are found again by replaying the
+// tap draws from seed0: `mask` marks the combined taps, each of which drew
+// one more number for its merge.  The common invPdf of y cancels.
+__device__ __forceinline__ float mis_weight(const FrameArgs& a, int x, int y, const Shading& s, const Res& res,
+                                         uint32_t M0, uint32_t seed0, uint32_t mask, int sel)
+{
+    if (!(res.wsum > 0.0f) || res.light >= a.L.n) return 0.0f;
+    const float4 la = a.L.a[res.light], lb = a.L.b[res.light];
+    const f3 Le = f3{lb.x, lb.y, lb.z};
+    const f3 Nl = face_normal(res.face);
+    const float p0 = phat(a.point, res.pos, Nl, s.sp, s.N, s.brdf, Le, la.w);
+    float den = (float)M0 * p0;
+    float psel = sel < 0 ? p0 : 0.0f;
+    const float rad = (float)(a.spatialRadius - 1u);
+    uint32_t rs = seed0;
+    for (uint32_t i = 0; i < a.spatialTaps && mask >> i; i++) {
+        int dx, dy;
+        tap_offset(rs, rad, dx, dy);
+        if (!((mask >> i) & 1u)) continue;
+        (void)rnd(rs);  // the merge's draw
+        const int nx = x + dx, ny = y + dy;
+        const size_t ni = (size_t)ny * a.W + nx;
+        const uint32_t nh = a.cur.hitw[ni];
+        const uint4 n0 = a.nbr[2 * ni];
+        const Shading t = shading_at(a, nx, ny, nh, __uint_as_float(n0.y));
+        float pj = phat(a.point, res.pos, Nl, t.sp, t.N, t.brdf, Le, la.w);
+        if (pj > 0.0f) {
+            const f3 spo = f3{t.sp.x + 0.1f * t.N.x, t.sp.y + 0.1f * t.N.y, t.sp.z + 0.1f * t.N.z};
+            const f3 L = normalize(sub(res.pos, spo));
+            if (light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps)) pj = 0.0f;
+        }
+        den += (float)(n0.x & 0xffffffu) * pj;
+        if ((int)i == sel) psel = pj;
+    }
+    const float d = res.pHat * den;
+    const float W = (res.wsum * psel) / d;
+    return (d > 0.0f && isfinite(W)) ? W : 0.0f;
+}
+
+template <bool kMis>
 __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
 {
     int x, y;
@@ -376,14 +433,15 @@ __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
     if ((hitw & 0xffffffu) != 0u && a.spatial) {
         const float dist = a.cur.dist[idx];
         uint32_t seed = a.cur.seed[idx];
+        const uint32_t seed0 = seed, M0 = res.M;
+        uint32_t mask = 0u;  // combined taps (MIS merge)
+        int sel = -1;        // tap whose sample was taken last, -1 = own
         const Shading s = shading_at(a, x, y, hitw, dist);
         const float rad = (float)(a.spatialRadius - 1u);
         for (uint32_t i = 0; i < a.spatialTaps; i++) {
-            const float angle = rnd(seed) * kTwoPi;
-            const float radius = rad * sqrtf(rnd(seed)) + 1.0f;
-            float sn, cs;
-            sincos_det(angle, sn, cs);
-            const int nx = x + f2i(cs * radius), ny = y + f2i(sn * radius);
+            int dx, dy;
+            tap_offset(seed, rad, dx, dy);
+            const int nx = x + dx, ny = y + dy;
             if ((nx == x && ny == y) || nx < 0 || ny < 0 || nx >= (int)a.W || ny >= (int)a.H) continue;
             const size_t ni = (size_t)ny * a.W + nx;
             // the neighbour's 32-byte record: one sector instead of five planes
@@ -404,9 +462,16 @@ __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
             const float ph = phat(a.point, nr.pos, face_normal(nr.face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) *
                              nr.invPdf;
             res_reweigh(nr, ph, nr.M);
-            res_combine(res, nr, rnd(seed));
+            const float r0 = rnd(seed);
+            if (kMis) {
+                // UpdateReservoir's test: does the neighbour's sample take over?
+                if (r0 < nr.wsum / fmaxf(1e-5f, res.wsum + nr.wsum)) sel = (int)i;
+                mask |= 1u << i;
+            }
+            res_combine(res, nr, r0);
         }
         res_adjust(res, res.pHat);
+        if (kMis) res.W = mis_weight(a, x, y, s, res, M0, seed0, mask, sel);
         a.cur.seed[idx] = seed;
     }
     store_res(a.A, idx, res);
@@ -586,7 +651,10 @@ hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s)
     switch (stage) {
     case 0: hipLaunchKernelGGL(k_primary, g, dim3(256), 0, s, a); break;
     case 1: hipLaunchKernelGGL(k_initial, g, dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_spatial, g, dim3(256), 0, s, a); break;
+    case 2:
+        if (a.misSpatial) hipLaunchKernelGGL(k_spatial<true>, g, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_spatial<false>, g, dim3(256), 0, s, a);
+        break;
     case 3: hipLaunchKernelGGL(k_shade, g, dim3(256), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }

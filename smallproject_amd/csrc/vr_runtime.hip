@@ -292,6 +292,7 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
     a.doubleRadiance = (c->cfg.quirks & VR_QUIRK_DOUBLE_RADIANCE) ? 1u : 0u;
     a.shadowSteps = (c->cfg.quirks & VR_QUIRK_SHADOW_STEP_CAP) ? 128 : 999999;
     a.tileOrder = c->cfg.flags & 3u;
+    a.misSpatial = (c->cfg.flags & VR_FLAG_MIS_SPATIAL) ? 1u : 0u;
     a.candIdx = c->candIdx;
     a.candPdf = c->candPdf;
     a.clusterStart = c->clusterStart;
@@ -409,7 +410,7 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     if (cfg->brick_capacity == 0 || cfg->brick_capacity > (1u << 22)) return VR_ERR_ARG;
     if (cfg->source_pdf > VR_SOURCE_LEARNED || cfg->light_mode > VR_LIGHTS_POINT) return VR_ERR_ARG;
     if (cfg->emit_lut[0] != 0.0f) return VR_ERR_ARG;
-    if (cfg->flags > 0xffu) return VR_ERR_ARG;
+    if (cfg->flags & ~(0xffu | VR_FLAG_MIS_SPATIAL)) return VR_ERR_ARG;
     for (int s = 0; s < VR_NUM_STAGES; s++)
         if (((cfg->flags >> (2 * s)) & 3u) > VR_TILES_XCD_COLS) return VR_ERR_ARG;
     vr_ctx* c = new (std::nothrow) vr_ctx();
@@ -554,6 +555,9 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
     if (p->spatialRadius == 0 && p->spatial && p->spatialTaps)
         return fail(c, VR_ERR_ARG, "vr_render_frame: spatialRadius must be >= 1");
     if (!finite_camera(p)) return fail(c, VR_ERR_ARG, "vr_render_frame: non-finite camera parameters");
+    if ((c->cfg.flags & VR_FLAG_MIS_SPATIAL) && p->spatial && p->spatialTaps > VR_MIS_MAX_TAPS)
+        return fail(c, VR_ERR_ARG, "vr_render_frame: the MIS spatial merge takes at most " +
+                                       std::to_string(VR_MIS_MAX_TAPS) + " taps");
     const bool learned = c->cfg.source_pdf == VR_SOURCE_LEARNED;
     if (learned) {
         if (!c->mlpReady || c->nclusters == 0 || c->clusteredLights != c->nLights)

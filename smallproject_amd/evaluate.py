@@ -87,10 +87,11 @@ def render_sequence(world, renderer, pos, target, frames: int, gt_rgb, width: in
     return errs, float(s.sum()) / max(n, 1), [float(v) / max(n, 1) for v in s]
 
 
-def plain_mse(name: str, width=None, height=None, frames: int = 8, gt_spp: int = 4096, eval_frame=None) -> dict:
+def plain_mse(name: str, width=None, height=None, frames: int = 8, gt_spp: int = 4096, eval_frame=None,
+              flags: int = 0) -> dict:
     """MSE of the configuration's own (plain) method after `frames` frames
     at one camera of its path."""
-    sc = dataclasses.replace(CONFIGS[name], quirks=EVAL_QUIRKS)
+    sc = dataclasses.replace(CONFIGS[name], quirks=EVAL_QUIRKS, flags=flags)
     w = build_world(sc, width, height)
     W, H = w.cfg.width, w.cfg.height
     w.force_sync()
@@ -103,15 +104,20 @@ def plain_mse(name: str, width=None, height=None, frames: int = 8, gt_spp: int =
     errs, ms, stage = render_sequence(w, r, pos, tgt, frames, gt, W, H)
     w.close()
     return {"config": name, "resolution": [W, H], "camera_frame": f, "gt_spp": gt_spp, "gt_noise_mse": noise,
+            "spatial_merge": "mis" if flags & _lib.FLAG_MIS_SPATIAL else "biased",
             "frames": frames, "mse_per_frame": errs, "ms_per_frame": ms}
 
 
 def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_frames=None, eval_frames: int = 16,
                gt_spp: int = 4096, nsamples: int = 1 << 16, lr: float = 1e-3, train_steps: int = 16,
-               nclusters: int = 256, eval_frame=None, mix: float = 0.25) -> dict:
+               nclusters: int = 256, eval_frame=None, mix: float = 0.25, flags: int = 0) -> dict:
+    """Learned proposal vs the plain power source on one camera of the
+    configuration's path: online training from the plain method's frames,
+    then MSE per frame of both against the converged reference.  `flags`
+    (vr_config.flags) applies to both methods."""
     base = CONFIGS[name]
-    scP = dataclasses.replace(base, source=_lib.SOURCE_POWER, quirks=EVAL_QUIRKS)
-    scL = dataclasses.replace(base, source=_lib.SOURCE_LEARNED, quirks=EVAL_QUIRKS)
+    scP = dataclasses.replace(base, source=_lib.SOURCE_POWER, quirks=EVAL_QUIRKS, flags=flags)
+    scL = dataclasses.replace(base, source=_lib.SOURCE_LEARNED, quirks=EVAL_QUIRKS, flags=flags)
     wP = build_world(scP, width, height)
     W, H = wP.cfg.width, wP.cfg.height
     wP.force_sync()
@@ -147,6 +153,7 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
     k_eq = max(1, min(eval_frames, int(k_ref * msP / msL)))
     out = {
         "config": name, "resolution": [W, H], "camera_frame": f, "gt_spp": gt_spp, "gt_noise_mse": noise,
+        "spatial_merge": "mis" if flags & _lib.FLAG_MIS_SPATIAL else "biased",
         "clusters": int(clusters.max()) + 1, "mix": mix, "train_frames": T, "train_steps_per_frame": train_steps,
         "train_samples": nsamples, "lr": lr, "train_loss_first": losses[0], "train_loss_last": losses[-1],
         "plain": {"ms_per_frame": msP, "stage_ms": stP, "mse_per_frame": eP},
@@ -172,16 +179,18 @@ def main(argv=None) -> int:
     ap.add_argument("--train-steps", type=int, default=16)
     ap.add_argument("--mix", type=float, default=0.25, help="eps of the proposal's defensive power mixture")
     ap.add_argument("--eval-frames", type=int, default=16)
+    ap.add_argument("--mis", action="store_true", help="MIS-weighted (unbiased) spatial merge in every method")
     ap.add_argument("--json")
     args = ap.parse_args(argv)
+    flags = _lib.FLAG_MIS_SPATIAL if args.mis else 0
     results = []
     for name in args.configs.split(","):
         t0 = time.perf_counter()
         if CONFIGS[name].clusters:
             res = equal_time(name, args.width, args.height, args.train_frames, args.eval_frames, args.gt_spp,
-                             train_steps=args.train_steps, mix=args.mix)
+                             train_steps=args.train_steps, mix=args.mix, flags=flags)
         else:
-            res = plain_mse(name, args.width, args.height, 8, args.gt_spp)
+            res = plain_mse(name, args.width, args.height, 8, args.gt_spp, flags=flags)
         res["seconds"] = round(time.perf_counter() - t0, 1)
         print(json.dumps(res), flush=True)
         results.append(res)

@@ -74,7 +74,8 @@ def params_for_oracle(p) -> O.Params:
 
 
 def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, height: int | None = None,
-               quirks: int | None = None, source: int | None = None, check_every: bool = True, setup=None):
+               quirks: int | None = None, source: int | None = None, check_every: bool = True, setup=None,
+               flags: int | None = None):
     """Render `frames` frames of a configuration on the GPU and in the oracle
     with identical per-frame params; returns a Comparison per checked frame.
 
@@ -83,9 +84,10 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
     stage downstream of the proposal is compared exactly.  `setup(world,
     renderer)` runs once after the world is on the device."""
     import dataclasses
-    if quirks is not None or source is not None:
+    if quirks is not None or source is not None or flags is not None:
         sc = dataclasses.replace(sc, quirks=sc.quirks if quirks is None else quirks,
-                                 source=sc.source if source is None else source)
+                                 source=sc.source if source is None else source,
+                                 flags=sc.flags if flags is None else flags)
     w = configs.build_world(sc, width, height)
     W, H = w.cfg.width, w.cfg.height
     w.force_sync()
@@ -96,7 +98,8 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
     view = O.View(w.grid(), w.bricks())
     lut = np.array(list(w.cfg.emit_lut), np.float32)
     ocfg = O.make_config(W, H, sc.point, gfxexp=1, quirks=sc.quirks,
-                         source=O.SOURCE_SLOTS if learned else sc.source, lut=lut)
+                         source=O.SOURCE_SLOTS if learned else sc.source, lut=lut,
+                         spatial_mis=bool(sc.flags & _lib.FLAG_MIS_SPATIAL))
     of = O.Frames(ocfg, view, w.lights())
     results = []
     for f in range(frames):

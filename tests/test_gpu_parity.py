@@ -87,6 +87,34 @@ def test_physics_mode_and_power_source(gpu):
     assert all(c.ok for c in res), report
 
 
+@pytest.mark.parametrize("name,frames", [("c1_point_128", 4), ("c2_area_1024", 3)])
+def test_mis_spatial_merge_matches_oracle(gpu, name, frames):
+    """The MIS-weighted spatial merge (one visibility ray per combined
+    neighbour, VR_FLAG_MIS_SPATIAL) against the oracle's, with temporal
+    reuse carrying the unbiased weights into later frames."""
+    res = run_config(configs.CONFIGS[name], frames=frames, width=320, height=180, flags=_lib.FLAG_MIS_SPATIAL)
+    report = "\n".join(c.report() for c in res)
+    print(report)
+    assert all(c.ok for c in res), report
+
+
+def test_mis_spatial_merge_rejects_too_many_taps(gpu):
+    sc = configs.CONFIGS["c1_point_128"]
+    import dataclasses
+    w = configs.build_world(dataclasses.replace(sc, flags=_lib.FLAG_MIS_SPATIAL), 64, 48)
+    w.force_sync()
+    pos, tgt = sc.camera(0)
+    w.look_at(pos, tgt)
+    p = w.prepare_frame()
+    p.spatialTaps = _lib.MIS_MAX_TAPS + 1
+    with pytest.raises(_lib.VrError, match="at most 32 taps"):
+        w.renderer.render(p)
+    p.spatialTaps = _lib.MIS_MAX_TAPS
+    w.renderer.render(p)
+    w.renderer.sync()
+    w.close()
+
+
 @pytest.mark.parametrize("size", [(256, 144), (200, 120)])
 def test_tile_orders_render_identical_frames(gpu, size):
     """The workgroup->tile order is a bijection: every order renders the same

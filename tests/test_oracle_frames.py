@@ -91,3 +91,65 @@ def test_stream_length_bounds():
     assert np.all(fin["M"][hit] <= (1 + p.spatialTaps) * (nC + 20 * nC))
     assert np.all(init["M"][~hit] == 0) and np.all(fin["W"] >= 0)
     assert np.all(np.isfinite(fr.frame()))
+
+
+def _shadow_world(point):
+    w = World(default_config(48, 36, point=point, capacity=1 << 12))
+    for bz in range(16):
+        for bx in range(16):
+            w.add_brick(bx, 0, bz, FLOOR)
+    w.box(50, 8, 40, 54, 40, 90, 0xF888)  # a wall and a slab casting shadows
+    w.box(70, 8, 60, 90, 14, 64, 0xF8C8)
+    for (x, y, z, m) in ((40, 20, 64, 5), (80, 30, 50, 3), (60, 12, 80, 8), (30, 40, 30, 2)):
+        w.set(x, y, z, (m << 16) | 0xFFEB)
+    w.set_focal(focal_for_fov(60.0))
+    return w
+
+
+def _mean_images(point, spatial, mis, K, salt):
+    """K independent 1-spp frames (temporal reuse off, own R0 stream)."""
+    w = _shadow_world(point)
+    view = O.View(w.grid(), w.bricks())
+    # intensities x1000 keep every pHat*M above AdjustWeight's 1e-5 floor
+    # (reservoir.cl:40-43), which would otherwise bias the estimate itself
+    cfg = O.make_config(48, 36, point, spatial_mis=mis, lut=np.arange(256, dtype=np.float32) * 1000)
+    fr = O.Frames(cfg, view, w.lights())
+    imgs = np.empty((K, 48 * 36, 3))
+    for f in range(K):
+        w.look_at([40.0, 60.0, 30.0], [64.0, 8.0, 70.0])
+        w.set_next_r0((f * 2654435761 + salt * 40503 + 12345) & 0xFFFFFFFF)
+        p = O.Params.from_buffer_copy(bytes(w.prepare_frame()))
+        w.finish_frame()
+        p.temporal, p.spatial = 0, int(spatial)
+        fr.render(p, threads=8)
+        imgs[f] = fr.frame()[:, :3]
+    return imgs
+
+
+def _z2(X, Y):
+    """mean over pixels and channels of the squared z-score of the
+    difference of two sample means"""
+    K = len(X)
+    d = X.mean(0) - Y.mean(0)
+    v = X.var(0) / K + Y.var(0) / len(Y)
+    ok = v > 0
+    return float(np.mean(d[ok] ** 2 / v[ok]))
+
+
+@pytest.mark.parametrize("point", [True, False])
+def test_mis_spatial_merge_is_unbiased(point):
+    """Resampled importance sampling without reuse is unbiased, so the mean
+    of many independent frames with MIS-weighted spatial reuse must match
+    the mean without spatial reuse: per-pixel z-scores of the difference
+    average ~1.  The reference's biased merge (1/M normalisation,
+    cl/reservoir.cl:40-43 after cl/spatialSampling.cl) darkens pixels whose
+    neighbours see other lights or shadows, and fails the same test in this
+    scene.  A merge that drops a neighbour's visibility or M, or weights by
+    the wrong input, is biased too."""
+    K = 2048
+    plain = _mean_images(point, False, False, K, 1)
+    mis = _mean_images(point, True, True, K, 2)
+    biased = _mean_images(point, True, False, K, 3)
+    assert _z2(mis, plain) < 1.2
+    assert _z2(biased, plain) > 2.5
+    assert np.all(np.isfinite(mis))
