@@ -129,6 +129,93 @@ def quality(log_fn) -> dict:
             "learned_mse_per_frame": [round(v, 5) for v in r["learned"]["mse_per_frame"]]}
 
 
+def bench_batch(args) -> int:
+    """The batched-views workload: 16 scene instances of uneven resolution
+    and world size, one launch per stage over all views, the frame replayed
+    from a hipGraph; the persistent grid size is picked by timing live
+    frames.  Also times the same frames rendered view after view (one
+    launch per stage and view) for comparison."""
+    from smallproject_amd import Batch, configs
+    scs = configs.batch_instances()
+    t0 = time.perf_counter()
+    worlds = [configs.build_world(sc) for sc in scs]
+    for w in worlds:
+        w.force_sync()
+    log(f"bench: built and uploaded {len(worlds)} worlds in {time.perf_counter() - t0:.1f} s")
+    npx = sum(sc.width * sc.height for sc in scs)
+    frame_no = [0]
+
+    def place(f):
+        for sc, w in zip(scs, worlds):
+            pos, tgt = sc.camera(f % sc.frames)
+            w.look_at(pos, tgt)
+
+    def sequential(nframes):
+        for _ in range(nframes):
+            place(frame_no[0])
+            for w in worlds:
+                w.render()
+            frame_no[0] += 1
+        for w in worlds:
+            w.renderer.sync()
+
+    b = Batch(worlds)
+    b.set_graph(not args.no_graph)
+
+    def batched(nframes):
+        for _ in range(nframes):
+            place(frame_no[0])
+            b.render()
+            frame_no[0] += 1
+        b.sync()
+
+    batched(args.warmup)
+    # persistent grid: one workgroup per tile, or k per CU (256 CUs)
+    tune = {}
+    for blocks in (0, 256, 512, 1024, 2048):
+        b.set_blocks(blocks)
+        batched(2)
+        b.profile_begin()
+        batched(6)
+        ms, n = b.profile_read()
+        tune[blocks] = ms / max(n, 1)
+    best = min(tune, key=tune.get)
+    b.set_blocks(best)
+    log("bench: batch ms per frame by blocks " + ", ".join(f"{k}: {v:.3f}" for k, v in tune.items()) + f" -> {best}")
+    batched(2)
+    b.profile_begin()
+    t0 = time.perf_counter()
+    batched(args.steps)
+    elapsed = time.perf_counter() - t0
+    gpu_ms, nprof = b.profile_read()
+    sequential(2)
+    t1 = time.perf_counter()
+    sequential(args.steps)
+    seq_elapsed = time.perf_counter() - t1
+    fps = args.steps / elapsed
+    out = {
+        "metric": "batched views: frames/s of all views & Mrays/s",
+        "value": round(fps, 3), "unit": "batch frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural worlds, lights and camera paths)",
+        "config": {"workload": f"batch of {len(scs)} views, {npx} pixels per batch frame, area lights, M=32, "
+                               "5 taps r=30, temporal clamp 20x",
+                   "views": [{"world": sc.world_size, "resolution": [sc.width, sc.height], "lights": len(w.lights())}
+                             for sc, w in zip(scs, worlds)],
+                   "graph": not args.no_graph, "blocks": best},
+        "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
+        "gpu_ms_per_batch_frame": round(gpu_ms / max(nprof, 1), 4),
+        "blocks_ms": {str(k): round(v, 4) for k, v in tune.items()},
+        "view_after_view": {"ms_per_step": round(1000.0 * seq_elapsed / args.steps, 4),
+                            "fps": round(args.steps / seq_elapsed, 3)},
+    }
+    b.close()
+    for w in worlds:
+        w.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +229,7 @@ def main() -> int:
                     help="workgroup->tile order; auto times each on the first frames after warm-up")
     ap.add_argument("--spatial", default="biased", choices=["biased", "mis"],
                     help="spatial merge: the reference's (biased) or MIS-weighted with a visibility ray per neighbour")
+    ap.add_argument("--no-graph", action="store_true", help="batched views: launch without the hipGraph replay")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only the known-size copy used to scale HBM byte counters")
     args = ap.parse_args()
@@ -151,6 +239,8 @@ def main() -> int:
         return 2
     if args.calibrate:
         return calibrate()
+    if args.config == "c4_batch16":
+        return bench_batch(args)
 
     import dataclasses
 

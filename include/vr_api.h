@@ -151,6 +151,33 @@ int vr_set_lights(vr_ctx* ctx, const vr_light* lights, uint32_t n);
  * spatial reuse -> shading.  Asynchronous. */
 int vr_render_frame(vr_ctx* ctx, const vr_render_params* params);
 
+/* ---- batches of views (no counterpart in the reference) ----
+ * A batch renders the frames of up to VR_MAX_VIEWS contexts (views of
+ * different resolutions and worlds on one device) with one launch per stage
+ * over all their tiles.  Each view keeps its own world, lights, buffers and
+ * configuration; its frames are the ones vr_render_frame would render.  The
+ * batch runs on its own stream, after the work queued so far on each view's
+ * stream, and each view's later work (read-backs, commits) waits for the
+ * batch frame.  Destroy a batch before its views. */
+#define VR_MAX_VIEWS 64
+typedef struct vr_batch vr_batch;
+int vr_batch_create(vr_ctx* const* views, uint32_t n, vr_batch** out);
+int vr_batch_destroy(vr_batch* b);
+const char* vr_batch_last_error(const vr_batch* b);
+/* workgroups per stage launch: 0 = one per 16x16 tile, else a persistent
+ * grid of `blocks` workgroups striding over the tiles */
+int vr_batch_set_blocks(vr_batch* b, uint32_t blocks);
+/* capture each frame's launches (argument upload, proposals, four stages)
+ * into a hipGraph, replayed with new arguments every frame; re-captured
+ * when the launch shape or a view's proposal state changes */
+int vr_batch_set_graph(vr_batch* b, int enable);
+/* params[n]: one frame of every view (each view's own render params) */
+int vr_batch_render(vr_batch* b, const vr_render_params* params);
+int vr_batch_sync(vr_batch* b);
+/* GPU time of whole batch frames since vr_batch_profile_begin */
+int vr_batch_profile_begin(vr_batch* b);
+int vr_batch_profile_read(vr_batch* b, double* ms_sum, uint32_t* frames);
+
 /* Block until all work of the context has finished. */
 int vr_sync(vr_ctx* ctx);
 

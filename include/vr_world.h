@@ -76,6 +76,10 @@ int vrw_render(vrw_world* w);
  * advances the frame counters */
 void vrw_prepare_frame(vrw_world* w);
 void vrw_finish_frame(vrw_world* w);
+/* Render() up to the launch, for a batch of views (vr_batch_render with
+ * vrw_params() of each world, then vrw_finish_frame): creates the device
+ * context, uploads the light list if it changed, prepares the frame */
+int vrw_begin_frame(vrw_world* w);
 int vrw_commit(vrw_world* w);
 int vrw_force_sync(vrw_world* w);
 int vrw_stage_times(vrw_world* w, float ms[VR_NUM_STAGES]);

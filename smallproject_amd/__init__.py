@@ -7,4 +7,4 @@ The Python package only binds the shared library (smallproject_amd/lib/
 libvoxelrestir.so); every computation runs in it.
 """
 from ._lib import LIBPATH, VrConfig, VrError, VrLight, VrRenderParams, lib  # noqa: F401
-from .api import Renderer, World, camera_path, default_config, focal_for_fov, log_lut, terrain_height  # noqa: F401
+from .api import Batch, Renderer, World, camera_path, default_config, focal_for_fov, log_lut, terrain_height  # noqa: F401

@@ -24,6 +24,7 @@ QUIRKS_REFERENCE = 7
 TRACE_ALL, TRACE_OPAQUE = 0, 1
 TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS = 0, 1, 2
 FLAG_MIS_SPATIAL, MIS_MAX_TAPS = 1 << 8, 32
+MAX_VIEWS = 64
 NUM_STAGES = 4
 STAGE_NAMES = ("primary", "initial", "spatial", "shade")
 GRIDSIZE = 128 ** 3
@@ -87,6 +88,15 @@ SIGNATURES = {
     "vr_read_candidates": (C.c_int, [_P, _P, _P]),
     "vr_set_proposal_mix": (C.c_int, [_P, _f32]),
     "vr_read_cluster_tables": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "vr_batch_create": (C.c_int, [_P, _u32, _P]),
+    "vr_batch_destroy": (C.c_int, [_P]),
+    "vr_batch_last_error": (C.c_char_p, [_P]),
+    "vr_batch_set_blocks": (C.c_int, [_P, _u32]),
+    "vr_batch_set_graph": (C.c_int, [_P, C.c_int]),
+    "vr_batch_render": (C.c_int, [_P, _P]),
+    "vr_batch_sync": (C.c_int, [_P]),
+    "vr_batch_profile_begin": (C.c_int, [_P]),
+    "vr_batch_profile_read": (C.c_int, [_P, _P, _P]),
     # vr_world.h
     "vrw_create": (C.c_int, [_P, _P]),
     "vrw_destroy": (C.c_int, [_P]),
@@ -118,6 +128,7 @@ SIGNATURES = {
     "vrw_set_next_r0": (None, [_P, _u32]),
     "vrw_render": (C.c_int, [_P]),
     "vrw_prepare_frame": (None, [_P]),
+    "vrw_begin_frame": (C.c_int, [_P]),
     "vrw_finish_frame": (None, [_P]),
     "vrw_commit": (C.c_int, [_P]),
     "vrw_force_sync": (C.c_int, [_P]),

@@ -336,6 +336,14 @@ class World:
         C.memmove(C.byref(out), C.byref(self.params), C.sizeof(VrRenderParams))
         return out
 
+    def begin_frame(self) -> VrRenderParams:
+        """render() up to the launch (context, light upload, next frame's
+        params); returns a copy of the params for a batch frame."""
+        self._chk(lib().vrw_begin_frame(self.h), "begin_frame")
+        out = VrRenderParams()
+        C.memmove(C.byref(out), C.byref(self.params), C.sizeof(VrRenderParams))
+        return out
+
     def finish_frame(self):
         lib().vrw_finish_frame(self.h)
 
@@ -436,3 +444,67 @@ def camera_path(kind: int, seed: int, frame: int, nframes: int, size: int, scale
     check(lib().vrw_camera_path(kind, seed, frame, nframes, size, scale, terrain_seed, ptr(pos), ptr(tgt)), None,
           "camera_path")
     return pos, tgt
+
+
+class Batch:
+    """Frames of several worlds' views rendered together: one launch per
+    stage over the tiles of every view (vr_batch_*), optionally replayed
+    from a hipGraph.  Each world keeps its own device context; a batch frame
+    renders exactly the frame World.render() would."""
+
+    def __init__(self, worlds):
+        self.worlds = list(worlds)
+        n = len(self.worlds)
+        views = (C.c_void_p * n)(*[w.renderer.h for w in self.worlds])
+        h = C.c_void_p()
+        code = lib().vr_batch_create(views, n, C.byref(h))
+        if code != _lib.VR_OK:
+            raise _lib.VrError(code, "vr_batch_create: views must be distinct contexts on one device, "
+                                     f"at most {_lib.MAX_VIEWS}")
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vr_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _chk(self, code: int, what: str):
+        if code != _lib.VR_OK:
+            m = lib().vr_batch_last_error(self.h)
+            raise _lib.VrError(code, f"{what}: {m.decode() if m else ''}")
+
+    def set_blocks(self, blocks: int):
+        """Workgroups per stage launch (0: one per tile; else persistent)."""
+        self._chk(lib().vr_batch_set_blocks(self.h, blocks), "vr_batch_set_blocks")
+
+    def set_graph(self, enable: bool):
+        self._chk(lib().vr_batch_set_graph(self.h, int(enable)), "vr_batch_set_graph")
+
+    def render_params(self, params):
+        """One frame of every view with explicit render params (one per world)."""
+        assert len(params) == len(self.worlds)
+        arr = (VrRenderParams * len(params))(*params)
+        self._chk(lib().vr_batch_render(self.h, arr), "vr_batch_render")
+
+    def render(self):
+        """World.render() of every world, as one batch frame."""
+        params = [w.begin_frame() for w in self.worlds]
+        self.render_params(params)
+        for w in self.worlds:
+            w.finish_frame()
+
+    def sync(self):
+        self._chk(lib().vr_batch_sync(self.h), "vr_batch_sync")
+
+    def profile_begin(self):
+        self._chk(lib().vr_batch_profile_begin(self.h), "vr_batch_profile_begin")
+
+    def profile_read(self):
+        """(summed GPU ms of whole batch frames, frames) since profile_begin."""
+        ms = C.c_double()
+        n = C.c_uint32()
+        self._chk(lib().vr_batch_profile_read(self.h, C.byref(ms), C.byref(n)), "vr_batch_profile_read")
+        return ms.value, n.value
+

@@ -14,9 +14,13 @@ and render settings.  `build_world` constructs the host World (CPU only);
                  (sigma 16), emission log-uniform in [1e-2, 1e2]; area
                  lights, 1920x1080, 60-frame spline path; the source pdf is
                  `source` (power-proportional or the learned proposal)
+  batch_instances()  16 scene instances of uneven world size (256^3 -
+                 1024^3) and resolution (320x180 - 1920x1080) rendered as one
+                 batch of views
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 
 from . import _lib
@@ -56,12 +60,13 @@ class SceneConfig:
     quirks: int = _lib.QUIRKS_REFERENCE
     source: int = _lib.SOURCE_UNIFORM
     flags: int = 0                # vr_config.flags (VR_FLAG_MIS_SPATIAL)
+    capacity: int = _lib.BRICKCOUNT  # brick pool size (host and device)
     extra: dict = field(default_factory=dict)
 
     def config(self, width: int | None = None, height: int | None = None):
         lut = None if self.lut is None else log_lut(*self.lut)
         return default_config(width or self.width, height or self.height, point=self.point, quirks=self.quirks,
-                              source=self.source, lut=lut, flags=self.flags)
+                              source=self.source, lut=lut, flags=self.flags, capacity=self.capacity)
 
     def camera(self, frame: int):
         return camera_path(self.camera_kind, self.camera_seed, frame, self.frames, self.world_size, self.scale,
@@ -83,6 +88,34 @@ CONFIGS = {
         light_mode="clusters", clusters=256, cluster_sigma=16.0, cluster_y=(16, 128), emit_range=(1, 255),
         lut=(1e-2, 1e2), camera_kind=1, camera_seed=12, source=_lib.SOURCE_POWER),
 }
+
+
+BATCH_RESOLUTIONS = ((320, 180), (640, 360), (1280, 720), (1920, 1080))
+BATCH_WORLDS = (256, 512, 1024)
+
+
+def batch_instances(n: int = 16, seed: int = 8, frames: int = 30) -> list[SceneConfig]:
+    """The batched-views configuration: n independent scene instances, each
+    a seeded heightfield + boxes world of edge 256, 512 or 1024 (heightfield
+    scale edge/128, 2,000 boxes per 1024^2 of area), a resolution from
+    BATCH_RESOLUTIONS, 256-16,384 emissive surface voxels (log-uniform count,
+    intensities log-uniform in [0.1, 10]), area lights and its own spline
+    camera path.  Drawn with numpy's PCG64 generator seeded with `seed`."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        W, H = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+        size = BATCH_WORLDS[int(rng.integers(len(BATCH_WORLDS)))]
+        lights = int(round(math.exp(rng.uniform(math.log(256), math.log(16384)))))
+        base = int(rng.integers(1, 1 << 30))
+        out.append(SceneConfig(
+            name=f"c4_view{i}_{size}_{W}x{H}", width=W, height=H, point=False, fov=60.0, frames=frames,
+            world_size=size, scale=size // 128, terrain_seed=base, boxes=2000 * (size // 256) ** 2 // 16,
+            box_edge=(4, 32), box_seed=base + 1, lights=lights, light_seed=base + 2, light_mode="surface",
+            emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=base + 3,
+            capacity={256: 1 << 15, 512: 1 << 17, 1024: 1 << 19}[size]))
+    return out
 
 
 def build_world(sc: SceneConfig | str, width: int | None = None, height: int | None = None) -> World:

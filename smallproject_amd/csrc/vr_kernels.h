@@ -78,6 +78,25 @@ __device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
 }
 
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s);
+// Up to kChunkViews views rendered by one launch per stage: their
+// FrameArgs travel in the kernel argument (8 x 448 B + the tile table,
+// under the 4 KiB kernel-argument limit), so pointers in them are global
+// pointers to the compiler as in the one-view kernels.
+constexpr int kChunkViews = 8;
+struct ViewChunk {
+    FrameArgs v[kChunkViews];
+    uint32_t start[kChunkViews + 1];  // first 16x16 tile of each view; start[n] = total
+    uint32_t n;
+};
+static_assert(sizeof(ViewChunk) <= 4096, "kernel argument limit");
+
+// one launch of stage `stage` over the tiles of a chunk's views: one
+// workgroup per tile (blocks = 0) or a persistent grid of `blocks`
+hipError_t launch_stage_views(int stage, const ViewChunk& c, uint32_t blocks, bool anyMis, hipStream_t s);
+// the same launch as a graph kernel node (args[0] receives the argument
+// pointer; c and args must stay valid while the node is added or updated)
+hipError_t stage_views_node(int stage, const ViewChunk* c, uint32_t blocks, bool anyMis, void** args,
+                            hipKernelNodeParams* p);
 hipError_t launch_commit(const uint32_t* ids, const uint32_t* src, uint32_t* pool, uint32_t n, uint32_t cap,
                          hipStream_t s);
 hipError_t launch_uber(const uint32_t* grid, uint8_t* uber, hipStream_t s);

@@ -191,11 +191,8 @@ __device__ __forceinline__ uint32_t pick_light(const LightView& L, uint32_t sour
 // ---------------------------------------------------------------------------
 // k_primary: renderAlbedo (cl/ris.cl:295-389)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_primary(FrameArgs a)
+__device__ __forceinline__ void primary_px(const FrameArgs& a, int x, int y)
 {
-    int x, y;
-    tile_pixel(a.tileOrder, x, y);
-    if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t seed = wang_hash((uint32_t)x * 171u + (uint32_t)y * 1773u + a.R0);
     const f3 D = camera_ray(a.cam, x, y);
@@ -261,11 +258,8 @@ __device__ __forceinline__ Shading shading_at(const FrameArgs& a, int x, int y, 
 // ---------------------------------------------------------------------------
 // k_initial: candidates + visibility + temporal reuse (cl/ris.cl:20-124)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_initial(FrameArgs a)
+__device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
 {
-    int x, y;
-    tile_pixel(a.tileOrder, x, y);
-    if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t hitw = a.cur.hitw[idx];
     const float dist = a.cur.dist[idx];
@@ -422,11 +416,8 @@ __device__ __forceinline__ float mis_weight(const FrameArgs& a, int x, int y, co
 }
 
 template <bool kMis>
-__global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
+__device__ __forceinline__ void spatial_px(const FrameArgs& a, int x, int y)
 {
-    int x, y;
-    tile_pixel(a.tileOrder, x, y);
-    if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     Res res = load_res(a.B, idx);
     const uint32_t hitw = a.cur.hitw[idx];
@@ -480,11 +471,8 @@ __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
 // ---------------------------------------------------------------------------
 // k_shade: final shading (cl/ris.cl:155-293 with sky dome and editor off)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_shade(FrameArgs a)
+__device__ __forceinline__ void shade_px(const FrameArgs& a, int x, int y)
 {
-    int x, y;
-    tile_pixel(a.tileOrder, x, y);
-    if (x >= (int)a.W || y >= (int)a.H) return;
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t hitw = a.cur.hitw[idx];
     const uint32_t voxel = hitw & 0xffffffu;
@@ -528,6 +516,93 @@ __global__ __launch_bounds__(256) void k_shade(FrameArgs a)
         }
     }
     a.frame[idx] = make_float4(color.x, color.y, color.z, dist);
+}
+
+// ---------------------------------------------------------------------------
+// stage kernels: one view (FrameArgs by value, 16x16 tile per block in the
+// tile order of the stage), or many views in one persistent launch
+// ---------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ void one_view(const FrameArgs& a, F f)
+{
+    int x, y;
+    tile_pixel(a.tileOrder, x, y);
+    if (x < (int)a.W && y < (int)a.H) f(a, x, y);
+}
+
+// A chunk's views' 16x16 tiles are numbered view after view (start[v] =
+// first tile of view v, start[n] = total, row-major inside a view); block b
+// renders tiles b, b + gridDim.x, ...  The chunk is a kernel argument, so
+// the view of a tile is found by a uniform scan over start and its
+// FrameArgs (and the pointers in them) are read like any kernel argument.
+template <class F>
+__device__ __forceinline__ void view_tile(const ViewChunk& c, uint32_t t, F f)
+{
+    uint32_t v = 0;
+    while (v + 1 < c.n && c.start[v + 1] <= t) v++;
+    const FrameArgs& a = c.v[v];
+    const uint32_t lt = t - c.start[v];
+    const uint32_t tilesX = (a.W + 15u) >> 4;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int x = (int)(lt % tilesX) * 16 + (wv & 1) * 8 + (lane & 7);
+    const int y = (int)(lt / tilesX) * 16 + (wv >> 1) * 8 + (lane >> 3);
+    if (x < (int)a.W && y < (int)a.H) f(a, x, y);
+}
+
+// kPersistent: gridDim.x < total, each block loops; otherwise one tile per block
+template <bool kPersistent, class F>
+__device__ __forceinline__ void many_views(const ViewChunk& c, F f)
+{
+    const uint32_t total = c.start[c.n];
+    if (!kPersistent) {
+        if (blockIdx.x < total) view_tile(c, blockIdx.x, f);
+        return;
+    }
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) view_tile(c, t, f);
+}
+
+__global__ __launch_bounds__(256) void k_primary(FrameArgs a)
+{
+    one_view(a, [](const FrameArgs& v, int x, int y) { primary_px(v, x, y); });
+}
+__global__ __launch_bounds__(256) void k_initial(FrameArgs a)
+{
+    one_view(a, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
+}
+template <bool kMis>
+__global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
+{
+    one_view(a, [](const FrameArgs& v, int x, int y) { spatial_px<kMis>(v, x, y); });
+}
+__global__ __launch_bounds__(256) void k_shade(FrameArgs a)
+{
+    one_view(a, [](const FrameArgs& v, int x, int y) { shade_px(v, x, y); });
+}
+
+template <bool kPersistent>
+__global__ __launch_bounds__(256) void k_primary_views(ViewChunk c)
+{
+    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { primary_px(v, x, y); });
+}
+template <bool kPersistent>
+__global__ __launch_bounds__(256) void k_initial_views(ViewChunk c)
+{
+    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
+}
+// kAnyMis: some view merges with MIS weights (the kernel then carries the
+// MIS variant's registers)
+template <bool kPersistent, bool kAnyMis>
+__global__ __launch_bounds__(256) void k_spatial_views(ViewChunk c)
+{
+    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) {
+        if (kAnyMis && v.misSpatial) spatial_px<true>(v, x, y);
+        else spatial_px<false>(v, x, y);
+    });
+}
+template <bool kPersistent>
+__global__ __launch_bounds__(256) void k_shade_views(ViewChunk c)
+{
+    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { shade_px(v, x, y); });
 }
 
 // ---------------------------------------------------------------------------
@@ -659,6 +734,60 @@ hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s)
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <bool kP>
+static const void* stage_views_kernel_t(int stage, bool anyMis)
+{
+    switch (stage) {
+    case 0: return reinterpret_cast<const void*>(&k_primary_views<kP>);
+    case 1: return reinterpret_cast<const void*>(&k_initial_views<kP>);
+    case 2:
+        return anyMis ? reinterpret_cast<const void*>(&k_spatial_views<kP, true>)
+                      : reinterpret_cast<const void*>(&k_spatial_views<kP, false>);
+    case 3: return reinterpret_cast<const void*>(&k_shade_views<kP>);
+    default: return nullptr;
+    }
+}
+
+static const void* stage_views_kernel(int stage, bool anyMis, bool persistent)
+{
+    return persistent ? stage_views_kernel_t<true>(stage, anyMis) : stage_views_kernel_t<false>(stage, anyMis);
+}
+
+static uint32_t chunk_grid(const ViewChunk& c, uint32_t blocks)
+{
+    const uint32_t total = c.start[c.n];
+    return blocks && blocks < total ? blocks : total;
+}
+
+hipError_t launch_stage_views(int stage, const ViewChunk& c, uint32_t blocks, bool anyMis, hipStream_t s)
+{
+    if (c.n == 0 || c.n > (uint32_t)kChunkViews) return hipErrorInvalidValue;
+    const uint32_t g = chunk_grid(c, blocks);
+    const void* k = stage_views_kernel(stage, anyMis, g < c.start[c.n]);
+    if (!k) return hipErrorInvalidValue;
+    if (g == 0) return hipSuccess;
+    void* args[] = {const_cast<ViewChunk*>(&c)};
+    return hipLaunchKernel(k, dim3(g), dim3(256), args, 0, s);
+}
+
+hipError_t stage_views_node(int stage, const ViewChunk* c, uint32_t blocks, bool anyMis, void** args,
+                            hipKernelNodeParams* p)
+{
+    if (c->n == 0 || c->n > (uint32_t)kChunkViews) return hipErrorInvalidValue;
+    const uint32_t g = chunk_grid(*c, blocks);
+    const void* k = stage_views_kernel(stage, anyMis, g < c->start[c->n]);
+    if (!k) return hipErrorInvalidValue;
+    args[0] = const_cast<ViewChunk*>(c);
+    *p = hipKernelNodeParams{};
+    p->func = const_cast<void*>(k);
+    p->gridDim = dim3(g);
+    p->blockDim = dim3(256);
+    p->sharedMemBytes = 0;
+    p->kernelParams = args;
+    p->extra = nullptr;
+    return hipSuccess;
 }
 
 hipError_t launch_commit(const uint32_t* ids, const uint32_t* src, uint32_t* pool, uint32_t n, uint32_t cap,

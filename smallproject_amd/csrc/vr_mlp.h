@@ -50,6 +50,20 @@ __device__ __forceinline__ int mlp_face(uint32_t hitw)
     return side + (((hitw >> 26) & 1u) ? 3 : 0);
 }
 
+// salt of the proposal's target jitter for a frame with random seed R0
+__host__ __device__ inline uint32_t proposal_salt(uint32_t R0) { return R0 * 0x9e3779b9u + 0x632be5abu; }
+
+// arguments of one k_proposal launch as a graph kernel node (the node
+// reads them through ptrs while it is added or updated)
+struct ProposalArgs {
+    FrameArgs a;
+    MlpView m;
+    uint32_t* candIdx;
+    float* candPdf;
+    uint32_t salt;
+    void* ptrs[5];
+};
+hipError_t proposal_node(ProposalArgs* pa, hipKernelNodeParams* p);
 hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint32_t* candIdx, float* candPdf, uint32_t salt,
                            hipStream_t s);
 hipError_t launch_mlp_probs(const MlpView& m, const float* x, uint32_t B, float* out, hipStream_t s);

@@ -232,19 +232,23 @@ __device__ __forceinline__ void softmax_infer(const MlpView& m, const float* sBi
 // cluster in proportion to its power, so the source pdf of light i is
 // m_c * pmf_i / P_c (eps = 1 gives the plain power source exactly).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32_t* __restrict__ candIdx,
-                                                  float* __restrict__ candPdf, uint32_t salt)
+struct ProposalLds {
+    InferLds w[4];
+    float bias[kMaxClusters];
+    float power[kMaxClusters];
+};
+
+__device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView& m, uint32_t* __restrict__ candIdx,
+                                              float* __restrict__ candPdf, uint32_t salt, int tx, int ty,
+                                              ProposalLds& L)
 {
-    __shared__ InferLds lds[4];
-    __shared__ float sBias[kMaxClusters];
-    __shared__ float sPower[kMaxClusters];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    InferLds& s = lds[wv];
+    InferLds& s = L.w[wv];
+    float* sBias = L.bias;
+    float* sPower = L.power;
     stage_bias(m, sBias);
     for (int c = threadIdx.x; c < kMaxClusters; c += 256) sPower[c] = m.clusterPower[c];
     // the wave's 64 pixels: an 8x8 tile (same map as the frame kernels)
-    int tx, ty;
-    tile_of_block(a.tileOrder, tx, ty);
     auto pixel_of = [&](int p, int& x, int& y) {
         x = tx * 16 + (wv & 1) * 8 + (p & 7);
         y = ty * 16 + (wv >> 1) * 8 + (p >> 3);
@@ -334,6 +338,16 @@ __global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32
             }
     }
 }
+
+__global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32_t* __restrict__ candIdx,
+                                                  float* __restrict__ candPdf, uint32_t salt)
+{
+    __shared__ ProposalLds L;
+    int tx, ty;
+    tile_of_block(a.tileOrder, tx, ty);
+    proposal_tile(a, m, candIdx, candPdf, salt, tx, ty, L);
+}
+
 
 // ---------------------------------------------------------------------------
 // probabilities for explicit inputs (test and evaluation hook), computed as
@@ -593,6 +607,20 @@ hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint32_t* candI
 {
     hipLaunchKernelGGL(k_proposal, dim3((a.W + 15) / 16, (a.H + 15) / 16), dim3(256), 0, s, a, m, candIdx, candPdf, salt);
     return hipGetLastError();
+}
+
+hipError_t proposal_node(ProposalArgs* pa, hipKernelNodeParams* p)
+{
+    pa->ptrs[0] = &pa->a, pa->ptrs[1] = &pa->m, pa->ptrs[2] = &pa->candIdx, pa->ptrs[3] = &pa->candPdf;
+    pa->ptrs[4] = &pa->salt;
+    *p = hipKernelNodeParams{};
+    p->func = reinterpret_cast<void*>(&k_proposal);
+    p->gridDim = dim3((pa->a.W + 15) / 16, (pa->a.H + 15) / 16);
+    p->blockDim = dim3(256);
+    p->sharedMemBytes = 0;
+    p->kernelParams = pa->ptrs;
+    p->extra = nullptr;
+    return hipSuccess;
 }
 
 hipError_t launch_mlp_probs(const MlpView& m, const float* x, uint32_t B, float* out, hipStream_t s)

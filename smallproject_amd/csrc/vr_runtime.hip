@@ -5,6 +5,7 @@
 // (template/world.cpp:2127-2483).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -544,28 +545,38 @@ int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
     return VR_OK;
 }
 
+// the checks of vr_render_frame (also applied to every view of a batch)
+static int check_frame(vr_ctx* c, const vr_render_params* p, const char* who)
+{
+    if (!c->committed) return fail(c, VR_ERR_STATE, std::string(who) + ": no world committed");
+    if (p->skyDomeSampling) return fail(c, VR_ERR_UNSUPPORTED, std::string(who) + ": sky dome sampling is not supported");
+    if (p->numberOfLights != c->nLights)
+        return fail(c, VR_ERR_ARG, std::string(who) + ": numberOfLights " + std::to_string(p->numberOfLights) +
+                                       " differs from the uploaded light count " + std::to_string(c->nLights));
+    if (p->spatialRadius == 0 && p->spatial && p->spatialTaps)
+        return fail(c, VR_ERR_ARG, std::string(who) + ": spatialRadius must be >= 1");
+    if (!finite_camera(p)) return fail(c, VR_ERR_ARG, std::string(who) + ": non-finite camera parameters");
+    if ((c->cfg.flags & VR_FLAG_MIS_SPATIAL) && p->spatial && p->spatialTaps > VR_MIS_MAX_TAPS)
+        return fail(c, VR_ERR_ARG, std::string(who) + ": the MIS spatial merge takes at most " +
+                                       std::to_string(VR_MIS_MAX_TAPS) + " taps");
+    if (c->cfg.source_pdf == VR_SOURCE_LEARNED) {
+        if (!c->mlpReady || c->nclusters == 0 || c->clusteredLights != c->nLights)
+            return fail(c, VR_ERR_STATE, std::string(who) + ": the learned source needs vr_set_light_clusters for the "
+                                         "current light list");
+        if (p->numberOfCandidates > (uint32_t)kCandidates)
+            return fail(c, VR_ERR_ARG, std::string(who) + ": the learned source draws at most 32 candidates");
+    }
+    return VR_OK;
+}
+
 int vr_render_frame(vr_ctx* c, const vr_render_params* p)
 {
     if (!c || !p) return VR_ERR_ARG;
-    if (!c->committed) return fail(c, VR_ERR_STATE, "vr_render_frame: no world committed");
-    if (p->skyDomeSampling) return fail(c, VR_ERR_UNSUPPORTED, "vr_render_frame: sky dome sampling is not supported");
-    if (p->numberOfLights != c->nLights)
-        return fail(c, VR_ERR_ARG, "vr_render_frame: numberOfLights " + std::to_string(p->numberOfLights) +
-                                       " differs from the uploaded light count " + std::to_string(c->nLights));
-    if (p->spatialRadius == 0 && p->spatial && p->spatialTaps)
-        return fail(c, VR_ERR_ARG, "vr_render_frame: spatialRadius must be >= 1");
-    if (!finite_camera(p)) return fail(c, VR_ERR_ARG, "vr_render_frame: non-finite camera parameters");
-    if ((c->cfg.flags & VR_FLAG_MIS_SPATIAL) && p->spatial && p->spatialTaps > VR_MIS_MAX_TAPS)
-        return fail(c, VR_ERR_ARG, "vr_render_frame: the MIS spatial merge takes at most " +
-                                       std::to_string(VR_MIS_MAX_TAPS) + " taps");
-    const bool learned = c->cfg.source_pdf == VR_SOURCE_LEARNED;
-    if (learned) {
-        if (!c->mlpReady || c->nclusters == 0 || c->clusteredLights != c->nLights)
-            return fail(c, VR_ERR_STATE, "vr_render_frame: the learned source needs vr_set_light_clusters for the "
-                                         "current light list");
-        if (p->numberOfCandidates > (uint32_t)kCandidates)
-            return fail(c, VR_ERR_ARG, "vr_render_frame: the learned source draws at most 32 candidates");
+    {
+        const int rc = check_frame(c, p, "vr_render_frame");
+        if (rc != VR_OK) return rc;
     }
+    const bool learned = c->cfg.source_pdf == VR_SOURCE_LEARNED;
     c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
     const FrameArgs a = frame_args(c, p);
     const int slot = (int)(c->frameNo % kRing);
@@ -580,7 +591,7 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         // the proposal runs after the primary hits and is timed with the
         // candidate stage
         if (s == VR_STAGE_INITIAL && learned)
-            HIPCHK(c, launch_proposal(as, mlp_view(c), c->candIdx, c->candPdf, p->R0 * 0x9e3779b9u + 0x632be5abu,
+            HIPCHK(c, launch_proposal(as, mlp_view(c), c->candIdx, c->candPdf, proposal_salt(p->R0),
                                       c->stream));
         HIPCHK(c, launch_stage(s, as, c->stream));
         HIPCHK(c, hipEventRecord(c->ev[slot][s + 1], c->stream));
@@ -645,6 +656,8 @@ int vr_stage_times(vr_ctx* c, float ms[VR_NUM_STAGES])
 {
     if (!c || !ms) return VR_ERR_ARG;
     if (!c->framed) return fail(c, VR_ERR_STATE, "vr_stage_times: no frame rendered");
+    if (c->lastSlot < 0)
+        return fail(c, VR_ERR_STATE, "vr_stage_times: the last frame was rendered in a batch (vr_batch_profile_read)");
     const int slot = c->lastSlot;
     HIPCHK(c, hipEventSynchronize(c->ev[slot][VR_NUM_STAGES]));
     for (int s = 0; s < VR_NUM_STAGES; s++)
@@ -960,6 +973,290 @@ int vr_ground_truth(vr_ctx* c, const vr_render_params* p, uint32_t spp, uint32_t
     (void)hipStreamSynchronize(c->stream);
     dfree(dacc);
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches of views: per stage one launch per chunk of up to 8 views
+// ---------------------------------------------------------------------------
+struct vr_batch {
+    std::vector<vr_ctx*> views;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    static constexpr int kSlots = 4;  // frames in flight, each with its own launch arguments
+    struct Slot {
+        std::vector<ViewChunk> chunks;       // the frame's per-view arguments
+        std::vector<ProposalArgs> props;     // one per learned view
+        std::vector<void*> chunkArgs;        // kernel-argument pointers of the graph nodes
+        hipGraph_t graph = nullptr;          // kept for its node handles
+        hipGraphExec_t exec = nullptr;
+        std::vector<hipGraphNode_t> nodes;   // in launch order
+        uint64_t key = 0;
+        hipEvent_t begin = nullptr, done = nullptr;
+        bool inFlight = false;
+    };
+    Slot slot[kSlots];
+    std::vector<uint32_t> learned;           // indices of the views with the learned source
+    hipEvent_t viewReady = nullptr;          // orders each view's stream before the batch
+    uint32_t blocks = 0;                     // workgroups per launch (0: one per tile)
+    bool graph = false;
+    bool anyMis = false;
+    uint64_t frameNo = 0;
+    bool profiling = false;
+    double profSum = 0.0;
+    uint32_t profFrames = 0;
+    std::string err;
+};
+
+namespace {
+
+int bfail(vr_batch* b, int code, const std::string& msg)
+{
+    if (b) b->err = msg;
+    return code;
+}
+
+#define BHIPCHK(b, call)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) return bfail(b, VR_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// the node list of a frame: per stage, the proposals before the candidate
+// stage, then one node per chunk
+template <class F>
+hipError_t for_each_node(vr_batch* b, vr_batch::Slot& sl, F f)
+{
+    const size_t nc = sl.chunks.size();
+    sl.chunkArgs.resize(nc * VR_NUM_STAGES);
+    size_t k = 0;
+    for (int st = 0; st < VR_NUM_STAGES; st++) {
+        if (st == VR_STAGE_INITIAL)
+            for (ProposalArgs& pa : sl.props) {
+                hipKernelNodeParams p;
+                hipError_t e = proposal_node(&pa, &p);
+                if (e == hipSuccess) e = f(k++, p);
+                if (e != hipSuccess) return e;
+            }
+        for (size_t c = 0; c < nc; c++) {
+            hipKernelNodeParams p;
+            hipError_t e = stage_views_node(st, &sl.chunks[c], b->blocks, b->anyMis, &sl.chunkArgs[st * nc + c], &p);
+            if (e == hipSuccess) e = f(k++, p);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+void drop_graph(vr_batch::Slot& sl)
+{
+    if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+    if (sl.graph) (void)hipGraphDestroy(sl.graph);
+    sl.exec = nullptr, sl.graph = nullptr;
+    sl.nodes.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int vr_batch_create(vr_ctx* const* views, uint32_t n, vr_batch** out)
+{
+    if (!views || !out || n == 0 || n > VR_MAX_VIEWS) return VR_ERR_ARG;
+    *out = nullptr;
+    for (uint32_t i = 0; i < n; i++) {
+        if (!views[i] || views[i]->device != views[0]->device) return VR_ERR_ARG;
+        for (uint32_t j = 0; j < i; j++)
+            if (views[j] == views[i]) return VR_ERR_ARG;  // a view twice would swap its buffers twice
+    }
+    vr_batch* b = new (std::nothrow) vr_batch();
+    if (!b) return VR_ERR_NOMEM;
+    b->views.assign(views, views + n);
+    b->device = views[0]->device;
+    for (uint32_t i = 0; i < n; i++) {
+        if (views[i]->cfg.flags & VR_FLAG_MIS_SPATIAL) b->anyMis = true;
+        if (views[i]->cfg.source_pdf == VR_SOURCE_LEARNED) b->learned.push_back(i);
+    }
+    // chunks of kChunkViews views with their tile tables (fixed: a view's
+    // resolution never changes)
+    const uint32_t nc = (n + kChunkViews - 1) / kChunkViews;
+    hipError_t e = hipSetDevice(b->device);
+    for (int k = 0; k < vr_batch::kSlots; k++) {
+        vr_batch::Slot& sl = b->slot[k];
+        sl.chunks.assign(nc, ViewChunk{});
+        for (uint32_t c = 0; c < nc; c++) {
+            ViewChunk& ch = sl.chunks[c];
+            std::memset(&ch, 0, sizeof(ch));
+            ch.n = std::min<uint32_t>(kChunkViews, n - c * kChunkViews);
+            for (uint32_t j = 0; j < ch.n; j++) {
+                const vr_config& cfg = views[c * kChunkViews + j]->cfg;
+                ch.start[j + 1] = ch.start[j] + ((cfg.width + 15) / 16) * ((cfg.height + 15) / 16);
+            }
+        }
+        sl.props.assign(b->learned.size(), ProposalArgs{});
+        if (e == hipSuccess) e = hipEventCreate(&sl.begin);
+        if (e == hipSuccess) e = hipEventCreate(&sl.done);
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->viewReady, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        vr_batch_destroy(b);
+        return VR_ERR_HIP;
+    }
+    *out = b;
+    return VR_OK;
+}
+
+int vr_batch_destroy(vr_batch* b)
+{
+    if (!b) return VR_ERR_ARG;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (vr_batch::Slot& sl : b->slot) {
+        drop_graph(sl);
+        if (sl.begin) (void)hipEventDestroy(sl.begin);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    }
+    if (b->viewReady) (void)hipEventDestroy(b->viewReady);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return VR_OK;
+}
+
+const char* vr_batch_last_error(const vr_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int vr_batch_set_blocks(vr_batch* b, uint32_t blocks)
+{
+    if (!b) return VR_ERR_ARG;
+    b->blocks = blocks;
+    return VR_OK;
+}
+
+int vr_batch_set_graph(vr_batch* b, int enable)
+{
+    if (!b) return VR_ERR_ARG;
+    b->graph = enable != 0;
+    return VR_OK;
+}
+
+int vr_batch_render(vr_batch* b, const vr_render_params* params)
+{
+    if (!b || !params) return VR_ERR_ARG;
+    const uint32_t n = (uint32_t)b->views.size();
+    for (uint32_t i = 0; i < n; i++) {
+        vr_ctx* c = b->views[i];
+        if (check_frame(c, &params[i], "vr_batch_render") != VR_OK)
+            return bfail(b, VR_ERR_ARG, "view " + std::to_string(i) + ": " + c->err);
+    }
+    BHIPCHK(b, hipSetDevice(b->device));
+    vr_batch::Slot& sl = b->slot[b->frameNo % vr_batch::kSlots];
+    if (sl.inFlight) {
+        // the slot's launch arguments (and graph) are free once its frame ended
+        BHIPCHK(b, hipEventSynchronize(sl.done));
+        if (b->profiling) {
+            float ms = 0.0f;
+            BHIPCHK(b, hipEventElapsedTime(&ms, sl.begin, sl.done));
+            b->profSum += ms, b->profFrames++;
+        }
+        sl.inFlight = false;
+    }
+    for (uint32_t i = 0, li = 0; i < n; i++) {
+        vr_ctx* c = b->views[i];
+        c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
+        FrameArgs a = frame_args(c, &params[i]);
+        a.tileOrder = VR_TILES_LINEAR;
+        sl.chunks[i / kChunkViews].v[i % kChunkViews] = a;
+        if (c->cfg.source_pdf == VR_SOURCE_LEARNED) {
+            ProposalArgs& pa = sl.props[li++];
+            pa.a = a, pa.m = mlp_view(c), pa.candIdx = c->candIdx, pa.candPdf = c->candPdf;
+            pa.salt = proposal_salt(params[i].R0);
+        }
+        c->lastCam = a.cam;
+        c->lastSlot = -1;
+        c->frameNo++;
+        c->framed = true;
+        // uploads queued on the view's own stream (commits, lights) first
+        BHIPCHK(b, hipEventRecord(b->viewReady, c->stream));
+        BHIPCHK(b, hipStreamWaitEvent(b->stream, b->viewReady, 0));
+    }
+    BHIPCHK(b, hipEventRecord(sl.begin, b->stream));
+    if (b->graph) {
+        const uint64_t key = ((uint64_t)b->blocks << 32) | (b->anyMis ? 1u : 0u);
+        if (!sl.exec || sl.key != key) {
+            drop_graph(sl);
+            BHIPCHK(b, hipGraphCreate(&sl.graph, 0));
+            const hipError_t e = for_each_node(b, sl, [&](size_t k, const hipKernelNodeParams& p) {
+                hipGraphNode_t node = nullptr;
+                const hipGraphNode_t* dep = k ? &sl.nodes[k - 1] : nullptr;
+                const hipError_t r = hipGraphAddKernelNode(&node, sl.graph, dep, k ? 1 : 0, &p);
+                if (r == hipSuccess) sl.nodes.push_back(node);
+                return r;
+            });
+            if (e != hipSuccess) {
+                drop_graph(sl);
+                return bfail(b, VR_ERR_HIP, std::string("vr_batch_render: graph build: ") + hipGetErrorString(e));
+            }
+            const hipError_t e2 = hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0);
+            if (e2 != hipSuccess) {
+                sl.exec = nullptr;
+                drop_graph(sl);
+                return bfail(b, VR_ERR_HIP, std::string("vr_batch_render: instantiate: ") + hipGetErrorString(e2));
+            }
+            sl.key = key;
+        } else {
+            // this frame's arguments into the slot's nodes (the slot's
+            // previous launch has finished)
+            BHIPCHK(b, for_each_node(b, sl, [&](size_t k, const hipKernelNodeParams& p) {
+                        return hipGraphExecKernelNodeSetParams(sl.exec, sl.nodes[k], &p);
+                    }));
+        }
+        BHIPCHK(b, hipGraphLaunch(sl.exec, b->stream));
+    } else {
+        BHIPCHK(b, for_each_node(b, sl, [&](size_t, const hipKernelNodeParams& p) {
+                    return hipLaunchKernel(p.func, p.gridDim, p.blockDim, p.kernelParams, 0, b->stream);
+                }));
+    }
+    BHIPCHK(b, hipEventRecord(sl.done, b->stream));
+    sl.inFlight = true;
+    // later work of each view (read-backs, commits) follows the frame
+    for (vr_ctx* c : b->views) BHIPCHK(b, hipStreamWaitEvent(c->stream, sl.done, 0));
+    b->frameNo++;
+    return VR_OK;
+}
+
+int vr_batch_sync(vr_batch* b)
+{
+    if (!b) return VR_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    return VR_OK;
+}
+
+int vr_batch_profile_begin(vr_batch* b)
+{
+    if (!b) return VR_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    for (vr_batch::Slot& sl : b->slot) sl.inFlight = false;
+    b->profSum = 0.0, b->profFrames = 0;
+    b->profiling = true;
+    return VR_OK;
+}
+
+int vr_batch_profile_read(vr_batch* b, double* ms_sum, uint32_t* frames)
+{
+    if (!b || !ms_sum || !frames) return VR_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    for (vr_batch::Slot& sl : b->slot)
+        if (sl.inFlight) {
+            if (b->profiling) {
+                float ms = 0.0f;
+                BHIPCHK(b, hipEventElapsedTime(&ms, sl.begin, sl.done));
+                b->profSum += ms, b->profFrames++;
+            }
+            sl.inFlight = false;
+        }
+    *ms_sum = b->profSum;
+    *frames = b->profFrames;
+    return VR_OK;
 }
 
 }  // extern "C"

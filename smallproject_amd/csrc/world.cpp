@@ -459,15 +459,22 @@ void World::FinishFrame()
     params.restirtemporalframe = params.restirtemporalframe + 1;
 }
 
-void World::Render()
+bool World::BeginFrame()
 {
-    // World::Render, template/world.cpp:2127-2340
-    if (!EnsureContext()) return;
+    if (!EnsureContext()) return false;
     if (lightsDirty) {
-        if (Check(vr_set_lights(ctx, lights.data(), (uint32_t)lights.size()), "SetupLightBuffer") != VR_OK) return;
+        if (Check(vr_set_lights(ctx, lights.data(), (uint32_t)lights.size()), "SetupLightBuffer") != VR_OK)
+            return false;
         lightsDirty = false;
     }
     PrepareFrame();
+    return true;
+}
+
+void World::Render()
+{
+    // World::Render, template/world.cpp:2127-2340
+    if (!BeginFrame()) return;
     if (Check(vr_render_frame(ctx, &params), "Render") != VR_OK) return;
     FinishFrame();
 }

@@ -69,6 +69,9 @@ public:
 
     // per frame (template/world.cpp:2127-2483)
     void Render();
+    // Render() up to the launch: device context, light upload when the list
+    // changed, the next frame's params (for a batch of views)
+    bool BeginFrame();
     void Commit();
     // the two halves of Render() around the launch, for drivers that render
     // through another path: camera/reprojection/R0 set-up, frame counters

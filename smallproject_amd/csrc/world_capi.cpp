@@ -108,6 +108,11 @@ int vrw_render(vrw_world* w)
     return W(w)->LastStatus();
 }
 void vrw_prepare_frame(vrw_world* w) { W(w)->PrepareFrame(); }
+int vrw_begin_frame(vrw_world* w)
+{
+    W(w)->BeginFrame();
+    return W(w)->LastStatus();
+}
 void vrw_finish_frame(vrw_world* w) { W(w)->FinishFrame(); }
 int vrw_commit(vrw_world* w)
 {
