@@ -108,13 +108,22 @@ def measured_traffic(workload: str) -> dict:
 def quality(log_fn) -> dict:
     """The metric's image-quality half: equal-time MSE against a 4096-spp
     binary64 converged reference, learned proposal vs the plain power
-    source, on the many-light configuration at 1920x1080."""
+    source, on the many-light configuration at 1920x1080, with the
+    reference's (biased) spatial merge and with the MIS-weighted one."""
+    from smallproject_amd import _lib
+    out = {}
+    for merge, flags in (("biased", 0), ("mis", _lib.FLAG_MIS_SPATIAL)):
+        out[merge] = quality_one(log_fn, merge, flags)
+    return out
+
+
+def quality_one(log_fn, merge: str, flags: int) -> dict:
     from smallproject_amd.evaluate import equal_time
     t0 = time.perf_counter()
-    r = equal_time("c3_clusters_1024")
-    log_fn(f"bench: equal-time MSE evaluation in {time.perf_counter() - t0:.1f} s")
+    r = equal_time("c3_clusters_1024", flags=flags)
+    log_fn(f"bench: equal-time MSE evaluation ({merge} spatial merge) in {time.perf_counter() - t0:.1f} s")
     return {"workload": "c3_clusters_1024: 30,000 boxes on a floor, 65,536 emitters in 256 clusters, 1920x1080, "
-                        "M=32, biased spatial reuse, static camera from reset history",
+                        f"M=32, {merge} spatial merge, static camera from reset history",
             "reference": f"{r['gt_spp']}-spp binary64 ground truth (its own noise MSE {r['gt_noise_mse']:.2e})",
             "mix": r["mix"], "train": f"{r['train_frames']} frames x {r['train_steps_per_frame']} Adam steps on 2^16 "
                                       "samples/frame from the plain method's reservoirs",
