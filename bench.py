@@ -264,8 +264,13 @@ def main() -> int:
     w.force_sync()
     r = w.renderer
     frames_total = args.warmup + args.steps
+    dynamic = sc.light_speed > 0 or sc.edit_permille > 0
+    if dynamic:
+        configs.start_motion(sc, w)
 
     def step(f: int) -> None:
+        if dynamic:  # the scene's motion on the device is part of the frame
+            configs.scene_step(sc, w, f)
         pos, tgt = sc.camera(f % sc.frames)
         w.look_at(pos, tgt)
         w.render()
@@ -314,8 +319,11 @@ def main() -> int:
         "config": {"workload": f"{sc.name}: plain Voxel ReSTIR, area lights, {sc.width}x{sc.height}, "
                                f"1024^3 world, {len(w.lights())} emitters, M={sc.candidates}, "
                                f"{sc.taps} taps r={sc.radius}, temporal clamp {sc.temporal_clamp}x"
-                               + (", MIS spatial merge" if args.spatial == "mis" else ""),
-                   "spatial_merge": args.spatial,
+                               + (", MIS spatial merge" if sc.flags & _lib.FLAG_MIS_SPATIAL else "")
+                               + (f", {len(w.lights())} emitters moving at up to {sc.light_speed:g} voxels/frame, "
+                                  f"{sc.edit_permille / 10:g}% of bricks edited per frame on the device"
+                                  if dynamic else ""),
+                   "spatial_merge": "mis" if sc.flags & _lib.FLAG_MIS_SPATIAL else "biased",
                    "resolution": [sc.width, sc.height], "frames_in_path": sc.frames,
                    "tile_order": tile_order,
                    "tile_order_stage_ms": None if tune_ms is None else

@@ -151,6 +151,35 @@ int vr_set_lights(vr_ctx* ctx, const vr_light* lights, uint32_t n);
  * spatial reuse -> shading.  Asynchronous. */
 int vr_render_frame(vr_ctx* ctx, const vr_render_params* params);
 
+/* ---- scene motion on the device (no counterpart in the reference, whose
+ * worlds change through host edits and World::Commit) ----
+ * These change the device copy of the world only: brick payloads and light
+ * records.  The host World keeps its own copy (vr_read_lights and
+ * vr_read_bricks return the device's).  Grid cells never change kind, so the
+ * top grid and uber grid stay as committed. */
+/* velocity[3n] in voxels per frame for the n lights of the current list
+ * (|component| <= 64); only lights of size 1 move.  Each light's sub-voxel
+ * position restarts at its voxel centre.  vr_set_lights drops the motion. */
+int vr_set_light_motion(vr_ctx* ctx, const float* velocity, uint32_t n);
+/* One frame of motion, queued on the context's stream:
+ *  - brick edits: brick id < high_water is edited when
+ *    wang_hash(id*0x9e3779b1 ^ seed ^ frame*0x85ebca77) % 1000 < edit_permille;
+ *    a walker from a hashed voxel takes walk_steps one-voxel xorshift steps
+ *    inside the brick, toggling each voxel it lands on between empty and an
+ *    opaque hashed colour (emissive voxels are never touched);
+ *  - lights: position += velocity.  A light whose voxel changes moves when the
+ *    new voxel is empty, lies in a brick cell, and no lower-indexed light
+ *    targets the same claim bucket (hash of the voxel, 65536 buckets): its
+ *    old voxel is cleared and its voxel word written at the new one.  Any
+ *    other change of voxel bounces it (velocity reversed, position reset to
+ *    its voxel centre).  Deterministic: it does not depend on thread timing. */
+int vr_scene_step(vr_ctx* ctx, uint32_t frame, uint32_t seed, uint32_t edit_permille, uint32_t walk_steps,
+                  uint32_t high_water);
+/* the device's light list (positions after motion), n = light count */
+int vr_read_lights(vr_ctx* ctx, vr_light* out, uint32_t n);
+/* payloads of n bricks (out[n*512]) from the device pool */
+int vr_read_bricks(vr_ctx* ctx, const uint32_t* ids, uint32_t n, uint32_t* out);
+
 /* ---- batches of views (no counterpart in the reference) ----
  * A batch renders the frames of up to VR_MAX_VIEWS contexts (views of
  * different resolutions and worlds on one device) with one launch per stage

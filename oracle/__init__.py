@@ -97,6 +97,7 @@ def lib() -> C.CDLL:
             "or_state_create": (P, [u32, u32]),
             "or_state_destroy": (None, [P]),
             "or_render_frame": (None, [P, P, P, P, P, P, i32]),
+            "or_scene_step": (None, [P, P, u32, P, P, P, u32, u32, u32, u32, u32, u32]),
             "or_ground_truth": (None, [P, P, P, u32, u32, P, i32, i32]),
             "or_world_create": (P, [u32]),
             "or_world_destroy": (None, [P]),
@@ -267,6 +268,34 @@ class View:
     @property
     def ref(self):
         return C.byref(self.s)
+
+
+class Motion:
+    """Scene motion on the oracle's copy of the world (the product's
+    vr_set_light_motion + vr_scene_step): `view`'s bricks and the `lights`
+    array (LIGHT_DTYPE, e.g. Frames.lights) change in place."""
+
+    def __init__(self, view: View, lights: np.ndarray, velocity: np.ndarray, capacity: int | None = None):
+        assert lights.dtype == LIGHT_DTYPE and lights.flags["C_CONTIGUOUS"]
+        # the pool the oracle holds (every brick id the grid names is below it)
+        capacity = view.bricks.size // 512 if capacity is None else capacity
+        self.view, self.lights, self.cap = view, lights, capacity
+        n = len(lights)
+        v = np.asarray(velocity, np.float32).reshape(n, 3)
+        self.vel = np.zeros((n, 4), np.float32)
+        self.vel[:, :3] = v
+        pos = lights["position"].astype(np.uint32)
+        size = lights["size"].astype(np.float32)
+        hs = np.float32(0.5) * size
+        self.pos = np.zeros((n, 4), np.float32)
+        self.pos[:, 0] = (pos & 1023).astype(np.float32) + hs
+        self.pos[:, 1] = (pos >> 20).astype(np.float32) + hs
+        self.pos[:, 2] = ((pos >> 10) & 1023).astype(np.float32) + hs
+        assert self.view.bricks.size >= capacity * 512
+
+    def step(self, frame: int, seed: int, permille: int, steps: int, high_water: int):
+        lib().or_scene_step(ptr(self.view.grid), ptr(self.view.bricks), self.cap, ptr(self.lights), ptr(self.pos),
+                            ptr(self.vel), len(self.lights), frame, seed, permille, steps, high_water)
 
 
 def trace_batch(view: View, O: np.ndarray, D: np.ndarray, steps: int = 999999, kind: int = TRACE_ALL):

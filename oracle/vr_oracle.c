@@ -1338,3 +1338,102 @@ void or_build_uber(const uint32_t* grid, uint8_t* uber)
                 uber[x + z * 32 + y * 1024] = empty ? 0 : 1;
             }
 }
+
+/* ------------------------------------------------------------------ */
+/* scene motion: the product's vr_scene_step (moving emitters and       */
+/* random-walk brick edits), restated sequentially.  Brick edits come   */
+/* first; light moves see the edited bricks.                             */
+/* ------------------------------------------------------------------ */
+static uint32_t or_to_voxel(float f)
+{
+    const float fl = floorf(f);
+    return !(fl > 0.0f) ? 0u : (fl >= 1023.0f ? 1023u : (uint32_t)fl);
+}
+
+static int or_voxel_slot(const uint32_t* grid, uint32_t cap, uint32_t x, uint32_t y, uint32_t z, size_t* slot)
+{
+    const uint32_t g = grid[(x >> 3) + (z >> 3) * 128u + (y >> 3) * 16384u];
+    if (!(g & 1u) || (g >> 1) >= cap) return 0;
+    *slot = (size_t)(g >> 1) * 512u + (x & 7u) + (y & 7u) * 8u + (z & 7u) * 64u;
+    return 1;
+}
+
+static void or_bounce(float* pos, float* vel, uint32_t i, uint32_t cx, uint32_t cy, uint32_t cz)
+{
+    vel[4 * i] = -vel[4 * i], vel[4 * i + 1] = -vel[4 * i + 1], vel[4 * i + 2] = -vel[4 * i + 2];
+    pos[4 * i] = (float)cx + 0.5f, pos[4 * i + 1] = (float)cy + 0.5f, pos[4 * i + 2] = (float)cz + 0.5f;
+}
+
+void or_scene_step(const uint32_t* grid, uint32_t* bricks, uint32_t cap, or_light* lights, float* pos, float* vel,
+                   uint32_t n, uint32_t frame, uint32_t seed, uint32_t permille, uint32_t steps,
+                   uint32_t high_water)
+{
+    /* edits: brick id < high_water with hash % 1000 < permille; a walker of
+     * `steps` xorshift steps toggling non-emissive voxels */
+    for (uint32_t id = 0; steps > 0 && id < high_water && id < cap; id++) {
+        const uint32_t h = or_wang_hash((id * 0x9e3779b1u) ^ seed ^ (frame * 0x85ebca77u));
+        if (h % 1000u >= permille) continue;
+        uint32_t st = or_wang_hash(h);
+        uint32_t c[3] = {st & 7u, (st >> 3) & 7u, (st >> 6) & 7u};
+        const uint32_t colour = 0xF000u | ((st >> 9) & 0xFFFu);
+        uint32_t* b = bricks + (size_t)id * 512u;
+        for (uint32_t k = 0; k < steps; k++) {
+            const uint32_t r = or_random_int(&st);
+            const uint32_t step = ((r >> 8) & 1u) ? 1u : 7u;
+            c[r % 3u] = (c[r % 3u] + step) & 7u;
+            uint32_t* v = b + c[0] + c[1] * 8u + c[2] * 64u;
+            if ((*v >> 16) & 255u) continue;
+            *v = *v ? 0u : colour;
+        }
+    }
+    /* lights: targets and claims (lowest index per bucket), then moves */
+    uint32_t* target = (uint32_t*)malloc((size_t)(n ? n : 1) * sizeof(uint32_t));
+    uint32_t* claim = (uint32_t*)malloc((size_t)65536 * sizeof(uint32_t));
+    if (!target || !claim) {
+        free(target), free(claim);
+        return;
+    }
+    for (uint32_t b = 0; b < 65536u; b++) claim[b] = 0xffffffffu;
+    for (uint32_t i = 0; i < n; i++) {
+        target[i] = 0xffffffffu;
+        if (lights[i].size != 1u) continue;
+        const uint32_t cur = lights[i].position;
+        const uint32_t cx = cur & 1023u, cz = (cur >> 10) & 1023u, cy = cur >> 20;
+        const float nx = pos[4 * i] + vel[4 * i], ny = pos[4 * i + 1] + vel[4 * i + 1];
+        const float nz = pos[4 * i + 2] + vel[4 * i + 2];
+        const uint32_t tx = or_to_voxel(nx), ty = or_to_voxel(ny), tz = or_to_voxel(nz);
+        if (tx == cx && ty == cy && tz == cz) {
+            pos[4 * i] = nx, pos[4 * i + 1] = ny, pos[4 * i + 2] = nz;
+            continue;
+        }
+        size_t slot = 0;
+        if (or_voxel_slot(grid, cap, tx, ty, tz, &slot) && bricks[slot] == 0u) {
+            const uint32_t t = tx | (tz << 10) | (ty << 20);
+            const uint32_t bk = or_wang_hash(t) & 65535u;
+            target[i] = t;
+            if (i < claim[bk]) claim[bk] = i;
+        } else {
+            or_bounce(pos, vel, i, cx, cy, cz);
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t t = target[i];
+        if (t == 0xffffffffu) continue;
+        const uint32_t cur = lights[i].position;
+        const uint32_t cx = cur & 1023u, cz = (cur >> 10) & 1023u, cy = cur >> 20;
+        if (claim[or_wang_hash(t) & 65535u] != i) {
+            or_bounce(pos, vel, i, cx, cy, cz);
+            continue;
+        }
+        const uint32_t tx = t & 1023u, tz = (t >> 10) & 1023u, ty = t >> 20;
+        size_t st = 0, sc = 0;
+        (void)or_voxel_slot(grid, cap, tx, ty, tz, &st);
+        if (or_voxel_slot(grid, cap, cx, cy, cz, &sc)) bricks[sc] = 0u;
+        bricks[st] = lights[i].voxel;
+        pos[4 * i] += vel[4 * i], pos[4 * i + 1] += vel[4 * i + 1], pos[4 * i + 2] += vel[4 * i + 2];
+        lights[i].position = t;
+    }
+    free(target);
+    free(claim);
+}
+

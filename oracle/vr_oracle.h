@@ -215,6 +215,13 @@ void or_render_frame(or_state* s, const or_config* c, const or_params* p,
 void or_ground_truth(const or_config* c, const or_params* p, const or_world_view* w,
                      uint32_t spp, uint32_t frame0, double* accum, int y0, int y1);
 
+/* ---- scene motion (the product's vr_scene_step; no reference counterpart):
+ * brick edits, then light moves, on the oracle's copy of the world.
+ * pos/vel hold 4 floats per light (xyz used); lights of size 1 move. ---- */
+void or_scene_step(const uint32_t* grid, uint32_t* bricks, uint32_t cap, or_light* lights, float* pos, float* vel,
+                   uint32_t n, uint32_t frame, uint32_t seed, uint32_t permille, uint32_t steps,
+                   uint32_t high_water);
+
 /* ---- world build semantics, template/world.h:293-606, world.cpp:47-683 ---- */
 typedef struct or_world or_world;
 or_world* or_world_create(uint32_t brick_capacity);

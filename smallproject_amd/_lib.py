@@ -88,6 +88,10 @@ SIGNATURES = {
     "vr_read_candidates": (C.c_int, [_P, _P, _P]),
     "vr_set_proposal_mix": (C.c_int, [_P, _f32]),
     "vr_read_cluster_tables": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "vr_set_light_motion": (C.c_int, [_P, _P, _u32]),
+    "vr_scene_step": (C.c_int, [_P, _u32, _u32, _u32, _u32, _u32]),
+    "vr_read_lights": (C.c_int, [_P, _P, _u32]),
+    "vr_read_bricks": (C.c_int, [_P, _P, _u32, _P]),
     "vr_batch_create": (C.c_int, [_P, _u32, _P]),
     "vr_batch_destroy": (C.c_int, [_P]),
     "vr_batch_last_error": (C.c_char_p, [_P]),

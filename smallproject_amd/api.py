@@ -123,6 +123,30 @@ class Renderer:
         self._chk(lib().vr_profile_read(self.h, ptr(s), C.byref(n)), "vr_profile_read")
         return s, n.value
 
+    # scene motion on the device
+    def set_light_motion(self, velocity: np.ndarray):
+        """Velocity [n, 3] (voxels per frame) of every light of the current
+        list; lights of size 1 move on each scene_step."""
+        v = np.ascontiguousarray(velocity, dtype=np.float32).reshape(-1, 3)
+        self._chk(lib().vr_set_light_motion(self.h, ptr(v), v.shape[0]), "vr_set_light_motion")
+
+    def scene_step(self, frame: int, seed: int, edit_permille: int, walk_steps: int, high_water: int):
+        """One frame of device-side motion: random-walk brick edits, then
+        light moves (vr_scene_step)."""
+        self._chk(lib().vr_scene_step(self.h, frame, seed, edit_permille, walk_steps, high_water), "vr_scene_step")
+
+    def read_lights(self, n: int) -> np.ndarray:
+        """The device's light list as a structured array (position, voxel, size)."""
+        out = np.zeros(n, dtype=[("position", "<u4"), ("voxel", "<u4"), ("size", "<u4")])
+        self._chk(lib().vr_read_lights(self.h, out.ctypes.data_as(C.c_void_p), n), "vr_read_lights")
+        return out
+
+    def read_bricks(self, ids) -> np.ndarray:
+        ids = np.ascontiguousarray(ids, dtype=np.uint32).reshape(-1)
+        out = np.zeros((ids.size, 512), np.uint32)
+        self._chk(lib().vr_read_bricks(self.h, ptr(ids), ids.size, ptr(out)), "vr_read_bricks")
+        return out
+
     # learned proposal
     def set_light_clusters(self, cluster_of_light: np.ndarray, nclusters: int):
         c = np.ascontiguousarray(cluster_of_light, np.uint32)

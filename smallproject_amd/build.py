@@ -20,7 +20,7 @@ INCLUDE = ROOT / "include"
 LIBDIR = PKG / "lib"
 LIBNAME = "libvoxelrestir.so"
 
-HIP_SOURCES = ["vr_kernels.hip", "vr_mlp.hip", "vr_runtime.hip"]
+HIP_SOURCES = ["vr_kernels.hip", "vr_mlp.hip", "vr_dynamics.hip", "vr_runtime.hip"]
 CXX_SOURCES = ["world.cpp", "world_capi.cpp", "scenes.cpp", "vxio.cpp", "nbtio.cpp", "imageio.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-ffp-contract=off", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}"]

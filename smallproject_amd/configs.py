@@ -14,6 +14,9 @@ and render settings.  `build_world` constructs the host World (CPU only);
                  (sigma 16), emission log-uniform in [1e-2, 1e2]; area
                  lights, 1920x1080, 60-frame spline path; the source pdf is
                  `source` (power-proportional or the learned proposal)
+  c5_dynamic_1024  configuration 2's world with 16,384 emitters moving at up
+                 to 3 voxels/frame and 5% of the bricks edited per frame on
+                 the device, MIS-weighted spatial merge, 120 frames
   batch_instances()  16 scene instances of uneven world size (256^3 -
                  1024^3) and resolution (320x180 - 1920x1080) rendered as one
                  batch of views
@@ -61,6 +64,14 @@ class SceneConfig:
     source: int = _lib.SOURCE_UNIFORM
     flags: int = 0                # vr_config.flags (VR_FLAG_MIS_SPATIAL)
     capacity: int = _lib.BRICKCOUNT  # brick pool size (host and device)
+    # scene motion on the device (vr_scene_step): light speeds uniform in
+    # [0, light_speed] voxels/frame in uniform directions (motion_seed);
+    # edit_permille of the bricks edited per frame by walk_steps-step walks
+    light_speed: float = 0.0
+    motion_seed: int = 0
+    edit_permille: int = 0
+    walk_steps: int = 0
+    edit_seed: int = 0
     extra: dict = field(default_factory=dict)
 
     def config(self, width: int | None = None, height: int | None = None):
@@ -87,6 +98,12 @@ CONFIGS = {
         scale=0, terrain_seed=0, floor=8, boxes=30000, box_edge=(2, 24), box_seed=7, lights=65536, light_seed=11,
         light_mode="clusters", clusters=256, cluster_sigma=16.0, cluster_y=(16, 128), emit_range=(1, 255),
         lut=(1e-2, 1e2), camera_kind=1, camera_seed=12, source=_lib.SOURCE_POWER),
+    "c5_dynamic_1024": SceneConfig(
+        name="c5_dynamic_1024", width=1920, height=1080, point=False, fov=60.0, frames=120, world_size=1024,
+        scale=8, terrain_seed=1, boxes=2000, box_edge=(4, 32), box_seed=4, lights=16384, light_seed=9,
+        light_mode="surface", emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=6,
+        flags=_lib.FLAG_MIS_SPATIAL, light_speed=3.0, motion_seed=9, edit_permille=50, walk_steps=32,
+        edit_seed=10),
 }
 
 
@@ -116,6 +133,28 @@ def batch_instances(n: int = 16, seed: int = 8, frames: int = 30) -> list[SceneC
             emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=base + 3,
             capacity={256: 1 << 15, 512: 1 << 17, 1024: 1 << 19}[size]))
     return out
+
+
+def light_velocities(n: int, speed: float, seed: int):
+    """[n, 3] float32 velocities: directions uniform on the sphere, speeds
+    uniform in [0, speed], from numpy's PCG64 generator seeded with `seed`."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = rng.normal(size=(n, 3))
+    d /= np.maximum(np.linalg.norm(d, axis=1, keepdims=True), 1e-12)
+    return (d * rng.uniform(0.0, speed, size=(n, 1))).astype(np.float32)
+
+
+def start_motion(sc: SceneConfig, world: World):
+    """Hand the configuration's light velocities to the world's device
+    context (after force_sync)."""
+    if sc.light_speed > 0:
+        world.renderer.set_light_motion(light_velocities(len(world.lights()), sc.light_speed, sc.motion_seed))
+
+
+def scene_step(sc: SceneConfig, world: World, frame: int):
+    """The configuration's device-side motion for one frame."""
+    world.renderer.scene_step(frame, sc.edit_seed, sc.edit_permille, sc.walk_steps, world.high_water())
 
 
 def build_world(sc: SceneConfig | str, width: int | None = None, height: int | None = None) -> World:

@@ -14,6 +14,7 @@
 
 #include "vr_api.h"
 #include "vr_kernels.h"
+#include "vr_dynamics.h"
 #include "vr_mlp.h"
 
 using namespace vr;
@@ -91,6 +92,13 @@ struct vr_ctx {
     float* chunkLoss = nullptr;
     uint32_t trainCap = 0;
     Camera lastCam{};             // camera of the last rendered frame
+    // scene motion on the device (allocated by vr_set_light_motion)
+    float4* lightPos = nullptr;
+    float4* lightVel = nullptr;
+    uint32_t* lightVoxel = nullptr;
+    uint32_t* lightTarget = nullptr;
+    uint32_t* claim = nullptr;    // [kClaimBuckets]
+    uint32_t motionCap = 0, motionN = 0;
 };
 
 namespace {
@@ -466,6 +474,7 @@ int vr_destroy(vr_ctx* c)
     dfree(c->clusterAlias), dfree(c->lightShare), dfree(c->clusterPower);
     dfree(c->candIdx), dfree(c->candPdf);
     dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
+    dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget), dfree(c->claim);
     for (int i = 0; i <= VR_NUM_STAGES; i++)
         for (int f = 0; f < kRing; f++)
             if (c->ev[f][i]) (void)hipEventDestroy(c->ev[f][i]);
@@ -530,6 +539,7 @@ int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
     }
     c->nLights = n;
     c->hostLights.assign(lights, lights + n);
+    c->motionN = 0;  // a new list has no motion until vr_set_light_motion
     if (n == 0) return VR_OK;
     HIPCHK(c, hipMemcpyAsync(c->la, ha.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->lb, hb.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
@@ -973,6 +983,102 @@ int vr_ground_truth(vr_ctx* c, const vr_render_params* p, uint32_t spp, uint32_t
     (void)hipStreamSynchronize(c->stream);
     dfree(dacc);
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// scene motion on the device
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int vr_set_light_motion(vr_ctx* c, const float* velocity, uint32_t n)
+{
+    if (!c || (n > 0 && !velocity)) return VR_ERR_ARG;
+    if (n != c->nLights)
+        return fail(c, VR_ERR_ARG, "vr_set_light_motion: " + std::to_string(n) + " velocities for " +
+                                       std::to_string(c->nLights) + " lights");
+    std::vector<float4> vel(n);
+    std::vector<uint32_t> vox(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const float vx = velocity[3 * i], vy = velocity[3 * i + 1], vz = velocity[3 * i + 2];
+        if (!std::isfinite(vx) || !std::isfinite(vy) || !std::isfinite(vz) || std::fabs(vx) > 64.0f ||
+            std::fabs(vy) > 64.0f || std::fabs(vz) > 64.0f)
+            return fail(c, VR_ERR_ARG, "vr_set_light_motion: velocity of light " + std::to_string(i) +
+                                           " is not finite or exceeds 64 voxels per frame");
+        vel[i] = make_float4(vx, vy, vz, 0.0f);
+        vox[i] = c->hostLights[i].voxel;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n > c->motionCap) {
+        dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget);
+        c->lightPos = c->lightVel = nullptr, c->lightVoxel = c->lightTarget = nullptr;
+        c->motionCap = 0;
+        if (dalloc(&c->lightPos, n) != hipSuccess || dalloc(&c->lightVel, n) != hipSuccess ||
+            dalloc(&c->lightVoxel, n) != hipSuccess || dalloc(&c->lightTarget, n) != hipSuccess)
+            return fail(c, VR_ERR_NOMEM, "vr_set_light_motion: device allocation failed");
+        c->motionCap = n;
+    }
+    if (!c->claim && dalloc(&c->claim, kClaimBuckets) != hipSuccess)
+        return fail(c, VR_ERR_NOMEM, "vr_set_light_motion: device allocation failed");
+    c->motionN = n;
+    if (n == 0) return VR_OK;
+    // positions start at the voxel centres the light records hold
+    HIPCHK(c, hipMemcpyAsync(c->lightPos, c->la, (size_t)n * 16, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->lightVel, vel.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->lightVoxel, vox.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
+}
+
+int vr_scene_step(vr_ctx* c, uint32_t frame, uint32_t seed, uint32_t edit_permille, uint32_t walk_steps,
+                  uint32_t high_water)
+{
+    if (!c) return VR_ERR_ARG;
+    if (!c->committed) return fail(c, VR_ERR_STATE, "vr_scene_step: no world committed");
+    if (edit_permille > 1000 || high_water > c->cfg.brick_capacity || walk_steps > 4096)
+        return fail(c, VR_ERR_ARG, "vr_scene_step: edit_permille <= 1000, walk_steps <= 4096 and high_water <= "
+                                   "brick capacity");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, launch_edit_bricks(c->bricks, high_water, frame, seed, edit_permille, walk_steps, c->stream));
+    if (c->motionN > 0 && c->motionN == c->nLights) {
+        const LightMotionView m{c->la, c->lb, c->lightPos, c->lightVel, c->lightVoxel, c->lightTarget, c->claim,
+                                c->motionN};
+        HIPCHK(c, launch_move_lights(m, c->grid, c->bricks, c->cfg.brick_capacity, c->stream));
+    }
+    return VR_OK;
+}
+
+int vr_read_lights(vr_ctx* c, vr_light* out, uint32_t n)
+{
+    if (!c || (n > 0 && !out)) return VR_ERR_ARG;
+    if (n != c->nLights) return fail(c, VR_ERR_ARG, "vr_read_lights: n must equal the light count");
+    if (n == 0) return VR_OK;
+    std::vector<float4> lb(n);
+    HIPCHK(c, hipMemcpyAsync(lb.data(), c->lb, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n; i++) {
+        out[i] = c->hostLights[i];
+        uint32_t packed;
+        std::memcpy(&packed, &lb[i].w, 4);
+        const uint32_t x = packed & 1023u, z = (packed >> 10) & 1023u, y = packed >> 20;
+        out[i].position = x + z * 1024u + y * 1024u * 1024u;
+    }
+    return VR_OK;
+}
+
+int vr_read_bricks(vr_ctx* c, const uint32_t* ids, uint32_t n, uint32_t* out)
+{
+    if (!c || (n > 0 && (!ids || !out))) return VR_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n; i++) {
+        if (ids[i] >= c->cfg.brick_capacity)
+            return fail(c, VR_ERR_ARG, "vr_read_bricks: brick id " + std::to_string(ids[i]) + " beyond capacity");
+        HIPCHK(c, hipMemcpyAsync(out + (size_t)i * VR_BRICKSIZE, c->bricks + (size_t)ids[i] * VR_BRICKSIZE,
+                                 VR_BRICKSIZE * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
 }
 
 }  // extern "C"

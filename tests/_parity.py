@@ -75,14 +75,16 @@ def params_for_oracle(p) -> O.Params:
 
 def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, height: int | None = None,
                quirks: int | None = None, source: int | None = None, check_every: bool = True, setup=None,
-               flags: int | None = None):
+               flags: int | None = None, before_frame=None):
     """Render `frames` frames of a configuration on the GPU and in the oracle
     with identical per-frame params; returns a Comparison per checked frame.
 
     With the learned source the oracle takes the GPU's candidate slots (light
     index and source pdf per pixel and slot) as its candidate lists, so every
     stage downstream of the proposal is compared exactly.  `setup(world,
-    renderer)` runs once after the world is on the device."""
+    renderer)` runs once after the world is on the device;
+    `before_frame(f, world, renderer, view, frames)` before each frame (the
+    oracle's world view and frame state)."""
     import dataclasses
     if quirks is not None or source is not None or flags is not None:
         sc = dataclasses.replace(sc, quirks=sc.quirks if quirks is None else quirks,
@@ -103,6 +105,8 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
     of = O.Frames(ocfg, view, w.lights())
     results = []
     for f in range(frames):
+        if before_frame is not None:
+            before_frame(f, w, r, view, of)
         pos, tgt = sc.camera(f)
         w.look_at(pos, tgt)
         p = w.prepare_frame()
