@@ -27,6 +27,7 @@ struct FrameArgs {
     ResPlanes A;  // final reservoirs: previous frame's on entry, this frame's on exit
     ResPlanes B;  // candidates + temporal output
     uint4* nbr;   // [2 per pixel] the B reservoir as the spatial pass's neighbours read it
+    uint4* tested; // [per pixel] the candidate pass's shadow ray: {light | visible << 31, pos bits}
     float4* frame;
     const float* emitLut;
     uint32_t W, H;

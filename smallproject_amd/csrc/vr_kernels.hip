@@ -53,6 +53,27 @@ __device__ __forceinline__ void store_neighbour(uint4* nbr, size_t i, const Res&
     nbr[2 * i + 1] = make_uint4(__float_as_uint(r.pos.x), __float_as_uint(r.pos.y), __float_as_uint(r.pos.z), r.light);
 }
 
+// The shadow ray k_initial traced for its candidate sample: {light |
+// visible << 31, pos.xyz bits}, or kNoTest.  A later shadow ray from the same
+// surface point to the same light and point is the same ray with the same
+// answer, so shading and the MIS merge look it up instead of tracing again.
+constexpr uint32_t kNoTest = 0xffffffffu;
+
+__device__ __forceinline__ uint4 tested_record(uint32_t light, f3 pos, bool visible)
+{
+    return make_uint4(light | (visible ? 0x80000000u : 0u), __float_as_uint(pos.x), __float_as_uint(pos.y),
+                      __float_as_uint(pos.z));
+}
+
+// 1 = visible, 0 = occluded, -1 = not traced for this sample
+__device__ __forceinline__ int tested_visibility(const uint4 t, uint32_t light, f3 pos)
+{
+    if (t.x == kNoTest || (t.x & 0x7fffffffu) != light || t.y != __float_as_uint(pos.x) ||
+        t.z != __float_as_uint(pos.y) || t.w != __float_as_uint(pos.z))
+        return -1;
+    return (int)(t.x >> 31);
+}
+
 // GridCoordinatesFromHit + Uint3CubeEquals (cl/tools.cl:452-472) behind
 // isLightOccluded (cl/ris.cl:1-17).  A shadow ray that finds no opaque voxel
 // within its step budget counts as occluded.
@@ -266,6 +287,7 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
     Res res;
     res.wsum = 0.0f, res.W = 0.0f, res.pHat = 0.0f, res.invPdf = 0.0f;
     res.pos = f3{0.0f, 0.0f, 0.0f}, res.light = 0u, res.M = 0u, res.face = 7u;
+    uint4 tested = make_uint4(kNoTest, 0u, 0u, 0u);
     if ((hitw & 0xffffffu) != 0u) {
         uint32_t seed = a.cur.seed[idx];
         const Shading s = shading_at(a, x, y, hitw, dist);
@@ -317,8 +339,9 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
             const float4 lb = a.L.b[res.light];
             const float4 la = a.L.a[res.light];
             const f3 L = normalize(sub(res.pos, spo));
-            if (light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps))
-                res.W = 0.0f, res.wsum = 0.0f;
+            const bool occ = light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
+            if (occ) res.W = 0.0f, res.wsum = 0.0f;
+            tested = tested_record(res.light, res.pos, !occ);
         }
         if (a.restirtemporalframe > 0u && a.temporal) {
             // ReprojectWorldPoint (cl/tools.cl:479-488)
@@ -354,6 +377,7 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
     }
     store_res(a.B, idx, res);
     store_neighbour(a.nbr, idx, res, hitw, dist);
+    a.tested[idx] = tested;
 }
 
 // ---------------------------------------------------------------------------
@@ -403,9 +427,14 @@ __device__ __forceinline__ float mis_weight(const FrameArgs& a, int x, int y, co
         const Shading t = shading_at(a, nx, ny, nh, __uint_as_float(n0.y));
         float pj = phat(a.point, res.pos, Nl, t.sp, t.N, t.brdf, Le, la.w);
         if (pj > 0.0f) {
-            const f3 spo = f3{t.sp.x + 0.1f * t.N.x, t.sp.y + 0.1f * t.N.y, t.sp.z + 0.1f * t.N.z};
-            const f3 L = normalize(sub(res.pos, spo));
-            if (light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps)) pj = 0.0f;
+            const int known = tested_visibility(a.tested[ni], res.light, res.pos);
+            if (known == 0) {
+                pj = 0.0f;
+            } else if (known < 0) {
+                const f3 spo = f3{t.sp.x + 0.1f * t.N.x, t.sp.y + 0.1f * t.N.y, t.sp.z + 0.1f * t.N.z};
+                const f3 L = normalize(sub(res.pos, spo));
+                if (light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps)) pj = 0.0f;
+            }
         }
         den += (float)(n0.x & 0xffffffu) * pj;
         if ((int)i == sel) psel = pj;
@@ -496,7 +525,11 @@ __device__ __forceinline__ void shade_px(const FrameArgs& a, int x, int y)
                     const f3 spo = f3{s.sp.x + 0.1f * s.N.x, s.sp.y + 0.1f * s.N.y, s.sp.z + 0.1f * s.N.z};
                     const float4 la = a.L.a[res.light], lb = a.L.b[res.light];
                     const f3 L = normalize(sub(res.pos, spo));
-                    if (!light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps)) {
+                    const int known = tested_visibility(a.tested[idx], res.light, res.pos);
+                    const bool visible =
+                        known >= 0 ? known == 1
+                                   : !light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
+                    if (visible) {
                         const f3 inc = phat_full(a.point, res.pos, face_normal(res.face), s.sp, s.N, s.brdf,
                                                  f3{lb.x, lb.y, lb.z}, la.w);
                         color.x += inc.x * res.W * res.invPdf;

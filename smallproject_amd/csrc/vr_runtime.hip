@@ -56,6 +56,7 @@ struct vr_ctx {
     uint32_t* rm[2] = {};
     float4* frame = nullptr;
     uint4* nbr = nullptr;     // [2 per pixel] neighbour records of the B reservoirs
+    uint4* tested = nullptr;  // [per pixel] the candidate pass's shadow-ray result
     int cur = 1;              // current hit buffer (swapped before each frame)
     bool committed = false;
     bool framed = false;
@@ -292,6 +293,7 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
     a.B = ResPlanes{c->ra[1], c->rb[1], c->rm[1]};
     a.frame = c->frame;
     a.nbr = c->nbr;
+    a.tested = c->tested;
     a.emitLut = c->emitLut;
     a.W = c->cfg.width, a.H = c->cfg.height;
     a.point = c->cfg.light_mode == VR_LIGHTS_POINT ? 1u : 0u;
@@ -433,7 +435,8 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 32768)) &&
            ok(dalloc(&c->bricks, (size_t)cfg->brick_capacity * VR_BRICKSIZE)) &&
            ok(dalloc(&c->stageIds, VR_MAXCOMMITS)) && ok(dalloc(&c->stageData, (size_t)VR_MAXCOMMITS * VR_BRICKSIZE)) &&
-           ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n)) && ok(dalloc(&c->nbr, 2 * n));
+           ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n)) && ok(dalloc(&c->nbr, 2 * n)) &&
+           ok(dalloc(&c->tested, n));
     for (int i = 0; good && i < 2; i++)
         good = ok(dalloc(&c->hitw[i], n)) && ok(dalloc(&c->dist[i], n)) && ok(dalloc(&c->seed[i], n)) &&
                ok(dalloc(&c->ra[i], n)) && ok(dalloc(&c->rb[i], n)) && ok(dalloc(&c->rm[i], n));
@@ -463,7 +466,7 @@ int vr_destroy(vr_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->grid), dfree(c->uber), dfree(c->bricks), dfree(c->stageIds), dfree(c->stageData);
     dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf), dfree(c->emitLut), dfree(c->frame);
-    dfree(c->nbr);
+    dfree(c->nbr), dfree(c->tested);
     for (int i = 0; i < 2; i++) {
         dfree(c->hitw[i]), dfree(c->dist[i]), dfree(c->seed[i]);
         dfree(c->ra[i]), dfree(c->rb[i]), dfree(c->rm[i]);
