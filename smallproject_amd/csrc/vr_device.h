@@ -20,7 +20,7 @@ constexpr float kTwoPi = 6.28318530717958647692528f;  // template/common.h:199
 struct WorldView {
     const uint32_t* __restrict__ grid;    // [128^3] cell: 0 empty, c<<1 solid, id<<1|1 brick
     const uint32_t* __restrict__ bricks;  // pool [id*512 + lx + ly*8 + lz*64]
-    const uint8_t* __restrict__ uber;     // [32^3] 1 = some of its 4^3 cells non-empty
+    const uint32_t* __restrict__ uber;    // [32^3 bits] cell x + z*32 + y*1024: some of its 4^3 cells non-empty
 };
 
 // Light record prepared by the host from struct Light (common.h:134-139):
@@ -169,6 +169,8 @@ __device__ __forceinline__ void sincos_det(float angle, float& s_out, float& c_o
 // ---------------------------------------------------------------------------
 enum TraceKind { kTraceAll = 0, kTraceOpaque = 1, kTraceThrough = 2 };
 
+__device__ __forceinline__ uint32_t uber_bit(const uint32_t* uber, uint32_t i) { return (uber[i >> 5] >> (i & 31u)) & 1u; }
+
 template <int KIND>
 __device__ __forceinline__ bool voxel_stops(uint32_t v, uint32_t through)
 {
@@ -217,7 +219,7 @@ __device__ uint32_t trace(const WorldView& w, f3 A, f3 B, int steps, uint32_t th
     float tmy = ((float)(((up >> 10) & 31u) + oy) - A.y * 0.03125f) * rV.y;
     float tmz = ((float)((up & 31u) + oz) - A.z * 0.03125f) * rV.z;
     float t = 0.0f;
-    uint32_t o = w.uber[(up >> 20) + ((up & 31u) << 5) + (((up >> 10) & 31u) << 10)];
+    uint32_t o = uber_bit(w.uber, (up >> 20) + ((up & 31u) << 5) + (((up >> 10) & 31u) << 10));
     while (true) {
         if ((steps -= 4) <= 0) break;
         if (KIND == kTraceThrough ? (o != 0u || through != 0u) : (o != 0u)) {
@@ -285,7 +287,7 @@ __device__ uint32_t trace(const WorldView& w, f3 A, f3 B, int steps, uint32_t th
         if (t == tmy) tmy += td.y, up += dy, last = 1;
         if (t == tmz) tmz += td.z, up += dz, last = 2;
         if (up & 0xfe0f83e0u) break;
-        o = w.uber[(up >> 20) + ((up & 31u) << 5) + (((up >> 10) & 31u) << 10)];
+        o = uber_bit(w.uber, (up >> 20) + ((up & 31u) << 5) + (((up >> 10) & 31u) << 10));
     }
     return 0u;
 }

@@ -100,7 +100,7 @@ hipError_t stage_views_node(int stage, const ViewChunk* c, uint32_t blocks, bool
                             hipKernelNodeParams* p);
 hipError_t launch_commit(const uint32_t* ids, const uint32_t* src, uint32_t* pool, uint32_t n, uint32_t cap,
                          hipStream_t s);
-hipError_t launch_uber(const uint32_t* grid, uint8_t* uber, hipStream_t s);
+hipError_t launch_uber(const uint32_t* grid, uint32_t* uber, hipStream_t s);
 hipError_t launch_trace_batch(const WorldView& w, const float* O, const float* D, uint32_t n, uint32_t kind,
                               int32_t steps, uint32_t* voxel, float* dist, uint32_t* side, hipStream_t s);
 hipError_t launch_ground_truth(const FrameArgs& a, uint32_t spp, uint32_t frame0, double* accum, hipStream_t s);

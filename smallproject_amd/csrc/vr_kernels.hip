@@ -555,12 +555,31 @@ __device__ __forceinline__ void shade_px(const FrameArgs& a, int x, int y)
 // stage kernels: one view (FrameArgs by value, 16x16 tile per block in the
 // tile order of the stage), or many views in one persistent launch
 // ---------------------------------------------------------------------------
-template <class F>
+// The uber grid's 4 KiB of bits, staged in LDS by every workgroup of the
+// frame kernels: the DDA's outermost level then reads LDS.
+__device__ __forceinline__ uint32_t* uber_lds()
+{
+    __shared__ uint32_t s[1024];
+    return s;
+}
+
+__device__ __forceinline__ void stage_uber(const uint32_t* __restrict__ src)
+{
+    uint32_t* dst = uber_lds();
+    for (uint32_t i = threadIdx.x; i < 1024u; i += 256u) dst[i] = src[i];
+    __syncthreads();
+}
+
+// kTraces: the stage traces rays (and so reads the staged uber grid)
+template <bool kTraces = true, class F>
 __device__ __forceinline__ void one_view(const FrameArgs& a, F f)
 {
+    if (kTraces) stage_uber(a.w.uber);
     int x, y;
     tile_pixel(a.tileOrder, x, y);
-    if (x < (int)a.W && y < (int)a.H) f(a, x, y);
+    FrameArgs b = a;
+    if (kTraces) b.w.uber = uber_lds();
+    if (x < (int)a.W && y < (int)a.H) f(b, x, y);
 }
 
 // A chunk's views' 16x16 tiles are numbered view after view (start[v] =
@@ -568,12 +587,21 @@ __device__ __forceinline__ void one_view(const FrameArgs& a, F f)
 // renders tiles b, b + gridDim.x, ...  The chunk is a kernel argument, so
 // the view of a tile is found by a uniform scan over start and its
 // FrameArgs (and the pointers in them) are read like any kernel argument.
+// tile t of the chunk; `staged` is the view whose uber grid LDS holds
+// (~0u: none yet).  Uniform control flow: every thread of the block runs
+// the staging barriers.
 template <class F>
-__device__ __forceinline__ void view_tile(const ViewChunk& c, uint32_t t, F f)
+__device__ __forceinline__ void view_tile(const ViewChunk& c, uint32_t t, uint32_t& staged, F f)
 {
     uint32_t v = 0;
     while (v + 1 < c.n && c.start[v + 1] <= t) v++;
-    const FrameArgs& a = c.v[v];
+    if (v != staged) {
+        if (staged != ~0u) __syncthreads();  // the previous view's tile is done with LDS
+        stage_uber(c.v[v].w.uber);
+        staged = v;
+    }
+    FrameArgs a = c.v[v];
+    a.w.uber = uber_lds();
     const uint32_t lt = t - c.start[v];
     const uint32_t tilesX = (a.W + 15u) >> 4;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -587,11 +615,12 @@ template <bool kPersistent, class F>
 __device__ __forceinline__ void many_views(const ViewChunk& c, F f)
 {
     const uint32_t total = c.start[c.n];
+    uint32_t staged = ~0u;
     if (!kPersistent) {
-        if (blockIdx.x < total) view_tile(c, blockIdx.x, f);
+        if (blockIdx.x < total) view_tile(c, blockIdx.x, staged, f);
         return;
     }
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) view_tile(c, t, f);
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) view_tile(c, t, staged, f);
 }
 
 __global__ __launch_bounds__(256) void k_primary(FrameArgs a)
@@ -605,7 +634,7 @@ __global__ __launch_bounds__(256) void k_initial(FrameArgs a)
 template <bool kMis>
 __global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
 {
-    one_view(a, [](const FrameArgs& v, int x, int y) { spatial_px<kMis>(v, x, y); });
+    one_view<kMis>(a, [](const FrameArgs& v, int x, int y) { spatial_px<kMis>(v, x, y); });
 }
 __global__ __launch_bounds__(256) void k_shade(FrameArgs a)
 {
@@ -654,11 +683,12 @@ __global__ __launch_bounds__(128) void k_commit(const uint32_t* __restrict__ ids
     pool[(size_t)id * 128u + threadIdx.x] = src[(size_t)b * 128u + threadIdx.x];
 }
 
-// updateUberGrid (cl/kernels.cl:498-513)
-__global__ __launch_bounds__(256) void k_uber(const uint32_t* __restrict__ grid, uint8_t* __restrict__ uber)
+// updateUberGrid (cl/kernels.cl:498-513), stored as bits: cell
+// i = x + z*32 + y*1024 is bit i & 31 of word i >> 5.  A wave covers 64
+// consecutive cells; its ballot holds their two words.
+__global__ __launch_bounds__(256) void k_uber(const uint32_t* __restrict__ grid, uint32_t* __restrict__ uber)
 {
-    const uint32_t task = blockIdx.x * 256u + threadIdx.x;
-    if (task >= 32768u) return;
+    const uint32_t task = blockIdx.x * 256u + threadIdx.x;  // 32768 tasks: the grid is exact
     const uint32_t x = task & 31u, z = (task >> 5) & 31u, y = task >> 10;
     uint32_t any = 0u;
     for (uint32_t b = 0; b < 4; b++)
@@ -666,7 +696,10 @@ __global__ __launch_bounds__(256) void k_uber(const uint32_t* __restrict__ grid,
             const uint4 row = *reinterpret_cast<const uint4*>(grid + (x * 4) + (z * 4 + c) * 128u + (y * 4 + b) * 16384u);
             any |= row.x | row.y | row.z | row.w;
         }
-    uber[x + z * 32u + y * 1024u] = any ? 1 : 0;
+    const uint64_t bits = __ballot(any != 0u);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (lane == 0u) uber[task >> 5] = (uint32_t)bits;
+    if (lane == 32u) uber[task >> 5] = (uint32_t)(bits >> 32);
 }
 
 // traceBatch (cl/kernels.cl:395-417) returning voxel, dist, side
@@ -832,7 +865,7 @@ hipError_t launch_commit(const uint32_t* ids, const uint32_t* src, uint32_t* poo
     return hipGetLastError();
 }
 
-hipError_t launch_uber(const uint32_t* grid, uint8_t* uber, hipStream_t s)
+hipError_t launch_uber(const uint32_t* grid, uint32_t* uber, hipStream_t s)
 {
     hipLaunchKernelGGL(k_uber, dim3(128), dim3(256), 0, s, grid, uber);
     return hipGetLastError();

@@ -35,7 +35,7 @@ struct vr_ctx {
     uint32_t profFrames = 0;
     // world
     uint32_t* grid = nullptr;       // [128^3]
-    uint8_t* uber = nullptr;        // [32^3]
+    uint32_t* uber = nullptr;       // [32^3 bits]
     uint32_t* bricks = nullptr;     // [brick_capacity * 512]
     uint32_t* stageIds = nullptr;   // [VR_MAXCOMMITS]
     uint32_t* stageData = nullptr;  // [VR_MAXCOMMITS * 512]
@@ -432,7 +432,7 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     bool good = ok(hipGetDevice(&c->device)) && ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int f = 0; good && f < kRing; f++)
         for (int i = 0; good && i <= VR_NUM_STAGES; i++) good = ok(hipEventCreate(&c->ev[f][i]));
-    good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 32768)) &&
+    good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 1024)) &&
            ok(dalloc(&c->bricks, (size_t)cfg->brick_capacity * VR_BRICKSIZE)) &&
            ok(dalloc(&c->stageIds, VR_MAXCOMMITS)) && ok(dalloc(&c->stageData, (size_t)VR_MAXCOMMITS * VR_BRICKSIZE)) &&
            ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n)) && ok(dalloc(&c->nbr, 2 * n)) &&
@@ -442,7 +442,7 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
                ok(dalloc(&c->ra[i], n)) && ok(dalloc(&c->rb[i], n)) && ok(dalloc(&c->rm[i], n));
     if (good) {
         good = ok(hipMemsetAsync(c->grid, 0, (size_t)VR_GRIDSIZE * 4, c->stream)) &&
-               ok(hipMemsetAsync(c->uber, 0, 32768, c->stream)) &&
+               ok(hipMemsetAsync(c->uber, 0, 4096, c->stream)) &&
                ok(hipMemsetAsync(c->bricks, 0, (size_t)cfg->brick_capacity * VR_BRICKSIZE * 4, c->stream)) &&
                ok(hipMemcpyAsync(c->emitLut, cfg->emit_lut, 256 * 4, hipMemcpyHostToDevice, c->stream)) &&
                ok(hipMemsetAsync(c->frame, 0, n * 16, c->stream));
