@@ -111,9 +111,20 @@ def quality(log_fn) -> dict:
     source, on the many-light configuration at 1920x1080, with the
     reference's (biased) spatial merge and with the MIS-weighted one."""
     from smallproject_amd import _lib
+    from smallproject_amd.evaluate import dynamic
     out = {}
     for merge, flags in (("biased", 0), ("mis", _lib.FLAG_MIS_SPATIAL)):
         out[merge] = quality_one(log_fn, merge, flags)
+    t0 = time.perf_counter()
+    d = dynamic("c5_dynamic_1024")
+    log_fn(f"bench: dynamic-scene learned/plain run in {time.perf_counter() - t0:.1f} s")
+    out["dynamic"] = {
+        "workload": "c5_dynamic_1024: 16,384 emitters moving up to 3 voxels/frame, 5% of bricks edited per frame, "
+                    "MIS spatial merge, 1920x1080, 120 frames along the path; learned proposal fine-tuned every 4 "
+                    "frames from the plain context, plain fallback when its MSE x frame time is worse "
+                    f"(checked every 16 frames against {d['gt_spp']}-spp references)",
+        "mean_mse_plain": d["mean_mse_plain"], "mean_mse_learned_context": d["mean_mse_learned_context"],
+        "checks": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in c.items()} for c in d["checks"]]}
     return out
 
 

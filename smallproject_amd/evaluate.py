@@ -35,7 +35,7 @@ import numpy as np
 
 from . import _lib
 from .api import image_error
-from .configs import CONFIGS, build_world
+from .configs import CONFIGS, build_world, scene_step, start_motion
 
 EVAL_QUIRKS = _lib.QUIRK_AXIS_ONLY_FACE | _lib.QUIRK_SHADOW_STEP_CAP
 
@@ -169,6 +169,91 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
     return out
 
 
+def dynamic(name: str = "c5_dynamic_1024", width=None, height=None, frames: int = 120, finetune_every: int = 4,
+            check_every: int = 16, gt_spp: int = 512, nsamples: int = 1 << 16, lr: float = 1e-3,
+            train_steps: int = 16, nclusters: int = 256, mix: float = 0.25, pretrain_frames: int = 32) -> dict:
+    """The dynamic configuration's learned-vs-plain run.  Two contexts render
+    the same moving scene (identical device motion, same camera path): the
+    plain power source, and the learned proposal fine-tuned every
+    `finetune_every` frames from the plain context's reservoirs.  Every
+    `check_every` frames both frames are compared with a converged reference
+    of the scene as it stands (gt_spp samples); when the learned method's
+    MSE x frame time is worse than the plain method's, the learned context
+    falls back to the plain source (mixture weight 1, which samples exactly
+    as the power source) until the next check, else it uses the proposal.
+    The network is first trained along `pretrain_frames` frames of the path
+    on a separate plain context."""
+    base = CONFIGS[name]
+    scP = dataclasses.replace(base, source=_lib.SOURCE_POWER, quirks=EVAL_QUIRKS)
+    scL = dataclasses.replace(base, source=_lib.SOURCE_LEARNED, quirks=EVAL_QUIRKS)
+    wP = build_world(scP, width, height)
+    W, H = wP.cfg.width, wP.cfg.height
+    wP.force_sync()
+    start_motion(scP, wP)
+    rP = wP.renderer
+    wL = build_world(scL, width, height)
+    wL.force_sync()
+    start_motion(scL, wL)
+    rL = wL.renderer
+    clusters = wL.cluster_lights(min(nclusters, len(wL.lights())), 8, 1)
+    rL.set_light_clusters(clusters, int(clusters.max()) + 1)
+    rL.set_proposal_mix(mix)
+    # pretraining on a static copy of the scene along the path
+    wT = build_world(scP, width, height)
+    wT.force_sync()
+    t0 = time.perf_counter()
+    for f in range(pretrain_frames):
+        pos, tgt = base.camera(f % base.frames)
+        wT.look_at(pos, tgt)
+        wT.render()
+        rL.mlp_train(nsamples, lr, train_steps, source=wT.renderer)
+    rL.sync()
+    wT.close()
+    _log(f"evaluate: pretrained on {pretrain_frames} frames in {time.perf_counter() - t0:.1f} s")
+    checks = []
+    using_proposal = True
+    msP = msL = 0.0
+    nP = nL = 0
+    for f in range(frames):
+        scene_step(scP, wP, f)
+        scene_step(scL, wL, f)
+        pos, tgt = base.camera(f % base.frames)
+        for w, r in ((wP, rP), (wL, rL)):
+            w.look_at(pos, tgt)
+            r.profile_begin()
+            w.render()
+            s_, n_ = r.profile_read()
+            if r is rP:
+                msP, nP = msP + float(s_.sum()), nP + n_
+            else:
+                msL, nL = msL + float(s_.sum()), nL + n_
+        if f % finetune_every == finetune_every - 1:
+            rL.mlp_train(nsamples, lr, train_steps, source=rP)
+        if f % check_every == check_every - 1:
+            wP.look_at(pos, tgt)
+            p = wP.prepare_frame()
+            gt, noise = converged_reference(rP, p, gt_spp)
+            eP = mse(gt, rP.frame(), W, H)
+            eL = mse(gt, rL.frame(), W, H)
+            tP, tL = msP / max(nP, 1), msL / max(nL, 1)
+            used = "learned" if using_proposal else "plain"
+            using_proposal = eL * tL <= eP * tP
+            rL.set_proposal_mix(mix if using_proposal else 1.0)
+            checks.append({"frame": f, "mse_plain": eP, "mse_learned_context": eL, "source_until_now": used,
+                           "ms_plain": tP, "ms_learned_context": tL, "gt_noise_mse": noise,
+                           "next_source": "learned" if using_proposal else "plain"})
+            _log(f"evaluate: frame {f}: MSE plain {eP:.4g} ({tP:.2f} ms), learned context {eL:.4g} ({tL:.2f} ms, "
+                 f"{used}) -> {checks[-1]['next_source']}")
+            msP = msL = 0.0
+            nP = nL = 0
+    wP.close()
+    wL.close()
+    return {"config": name, "resolution": [W, H], "frames": frames, "finetune_every": finetune_every,
+            "check_every": check_every, "gt_spp": gt_spp, "mix": mix, "checks": checks,
+            "mean_mse_plain": float(np.mean([c["mse_plain"] for c in checks])) if checks else None,
+            "mean_mse_learned_context": float(np.mean([c["mse_learned_context"] for c in checks])) if checks else None}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--configs", default="c1_point_128,c2_area_1024,c3_clusters_1024")
@@ -186,7 +271,9 @@ def main(argv=None) -> int:
     results = []
     for name in args.configs.split(","):
         t0 = time.perf_counter()
-        if CONFIGS[name].clusters:
+        if CONFIGS[name].light_speed > 0:
+            res = dynamic(name, args.width, args.height, gt_spp=min(args.gt_spp, 512), mix=args.mix)
+        elif CONFIGS[name].clusters:
             res = equal_time(name, args.width, args.height, args.train_frames, args.eval_frames, args.gt_spp,
                              train_steps=args.train_steps, mix=args.mix, flags=flags)
         else:

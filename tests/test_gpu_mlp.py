@@ -280,3 +280,20 @@ def test_learned_source_unbiased(gpu):
     a, b = means[_lib.SOURCE_POWER], means[_lib.SOURCE_LEARNED]
     se = np.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b))
     assert abs(a.mean() - b.mean()) <= 4 * se + 1e-3 * a.mean(), f"power {a.mean()} learned {b.mean()} se {se}"
+
+
+def test_dynamic_run_falls_back_by_its_rule(gpu):
+    """The dynamic configuration's learned-vs-plain run: each check compares
+    MSE x frame time and picks the next source by that rule; the learned
+    context's first period always uses the proposal."""
+    from smallproject_amd.evaluate import dynamic
+    r = dynamic("c5_dynamic_1024", 160, 96, frames=24, finetune_every=4, check_every=8, gt_spp=32,
+                nsamples=1 << 12, train_steps=2, pretrain_frames=2)
+    assert len(r["checks"]) == 3
+    assert r["checks"][0]["source_until_now"] == "learned"
+    for a, b in zip(r["checks"], r["checks"][1:]):
+        assert b["source_until_now"] == a["next_source"]
+    for c in r["checks"]:
+        better = c["mse_learned_context"] * c["ms_learned_context"] <= c["mse_plain"] * c["ms_plain"]
+        assert c["next_source"] == ("learned" if better else "plain")
+        assert c["mse_plain"] > 0 and c["mse_learned_context"] > 0
