@@ -623,38 +623,45 @@ __device__ __forceinline__ void many_views(const ViewChunk& c, F f)
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) view_tile(c, t, staged, f);
 }
 
-__global__ __launch_bounds__(256) void k_primary(FrameArgs a)
+// Occupancy targets (waves per SIMD) of the ray-tracing stages: their DDA
+// loops are bound by dependent-load latency, and more waves in flight hide
+// more of it than the few registers the compiler then spills cost
+// (measured on c2 at 1080p: candidates 1.83 -> 1.68 ms at 7 waves,
+// shading 0.58 -> 0.51 ms at 8, the MIS spatial merge 3.17 -> 2.59 ms at 8).
+#define VR_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
+
+__global__ __launch_bounds__(256) VR_WAVES(8) void k_primary(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { primary_px(v, x, y); });
 }
-__global__ __launch_bounds__(256) void k_initial(FrameArgs a)
+__global__ __launch_bounds__(256) VR_WAVES(7) void k_initial(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
 }
 template <bool kMis>
-__global__ __launch_bounds__(256) void k_spatial(FrameArgs a)
+__global__ __launch_bounds__(256) VR_WAVES(8) void k_spatial(FrameArgs a)
 {
     one_view<kMis>(a, [](const FrameArgs& v, int x, int y) { spatial_px<kMis>(v, x, y); });
 }
-__global__ __launch_bounds__(256) void k_shade(FrameArgs a)
+__global__ __launch_bounds__(256) VR_WAVES(8) void k_shade(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { shade_px(v, x, y); });
 }
 
 template <bool kPersistent>
-__global__ __launch_bounds__(256) void k_primary_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(8) void k_primary_views(ViewChunk c)
 {
     many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { primary_px(v, x, y); });
 }
 template <bool kPersistent>
-__global__ __launch_bounds__(256) void k_initial_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(7) void k_initial_views(ViewChunk c)
 {
     many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
 }
 // kAnyMis: some view merges with MIS weights (the kernel then carries the
 // MIS variant's registers)
 template <bool kPersistent, bool kAnyMis>
-__global__ __launch_bounds__(256) void k_spatial_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(8) void k_spatial_views(ViewChunk c)
 {
     many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) {
         if (kAnyMis && v.misSpatial) spatial_px<true>(v, x, y);
@@ -662,7 +669,7 @@ __global__ __launch_bounds__(256) void k_spatial_views(ViewChunk c)
     });
 }
 template <bool kPersistent>
-__global__ __launch_bounds__(256) void k_shade_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(8) void k_shade_views(ViewChunk c)
 {
     many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { shade_px(v, x, y); });
 }
