@@ -190,18 +190,23 @@ def bench_batch(args) -> int:
         b.sync()
 
     batched(args.warmup)
-    # persistent grid: one workgroup per tile, or k per CU (256 CUs)
+    # launch shape: views per chunk (chains that may overlap) x persistent
+    # grid (0 = one workgroup per tile, else k per CU of 256)
     tune = {}
-    for blocks in (0, 256, 512, 1024, 2048):
-        b.set_blocks(blocks)
-        batched(2)
-        b.profile_begin()
-        batched(6)
-        ms, n = b.profile_read()
-        tune[blocks] = ms / max(n, 1)
+    for chunk in (8, 4, 2, 1):
+        for blocks in ((0, 1024, 2048) if chunk == 8 else (0,)):
+            b.set_chunk_views(chunk)
+            b.set_blocks(blocks)
+            batched(2)
+            b.profile_begin()
+            batched(6)
+            ms, n = b.profile_read()
+            tune[(chunk, blocks)] = ms / max(n, 1)
     best = min(tune, key=tune.get)
-    b.set_blocks(best)
-    log("bench: batch ms per frame by blocks " + ", ".join(f"{k}: {v:.3f}" for k, v in tune.items()) + f" -> {best}")
+    b.set_chunk_views(best[0])
+    b.set_blocks(best[1])
+    log("bench: batch ms per frame by (views per chunk, blocks) " +
+        ", ".join(f"{k}: {v:.3f}" for k, v in tune.items()) + f" -> {best}")
     batched(2)
     b.profile_begin()
     t0 = time.perf_counter()
@@ -222,10 +227,10 @@ def bench_batch(args) -> int:
                                "5 taps r=30, temporal clamp 20x",
                    "views": [{"world": sc.world_size, "resolution": [sc.width, sc.height], "lights": len(w.lights())}
                              for sc, w in zip(scs, worlds)],
-                   "graph": not args.no_graph, "blocks": best},
+                   "graph": not args.no_graph, "views_per_chunk": best[0], "blocks": best[1]},
         "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
         "gpu_ms_per_batch_frame": round(gpu_ms / max(nprof, 1), 4),
-        "blocks_ms": {str(k): round(v, 4) for k, v in tune.items()},
+        "launch_shape_ms": {f"{k[0]} views/chunk, {k[1]} blocks": round(v, 4) for k, v in tune.items()},
         "view_after_view": {"ms_per_step": round(1000.0 * seq_elapsed / args.steps, 4),
                             "fps": round(args.steps / seq_elapsed, 3)},
     }

@@ -183,7 +183,8 @@ int vr_read_bricks(vr_ctx* ctx, const uint32_t* ids, uint32_t n, uint32_t* out);
 /* ---- batches of views (no counterpart in the reference) ----
  * A batch renders the frames of up to VR_MAX_VIEWS contexts (views of
  * different resolutions and worlds on one device) with one launch per stage
- * over all their tiles.  Each view keeps its own world, lights, buffers and
+ * per chunk of up to 8 views, over all their tiles.  Each view keeps its
+ * own world, lights, buffers and
  * configuration; its frames are the ones vr_render_frame would render.  The
  * batch runs on its own stream, after the work queued so far on each view's
  * stream, and each view's later work (read-backs, commits) waits for the
@@ -196,9 +197,13 @@ const char* vr_batch_last_error(const vr_batch* b);
 /* workgroups per stage launch: 0 = one per 16x16 tile, else a persistent
  * grid of `blocks` workgroups striding over the tiles */
 int vr_batch_set_blocks(vr_batch* b, uint32_t blocks);
-/* capture each frame's launches (argument upload, proposals, four stages)
- * into a hipGraph, replayed with new arguments every frame; re-captured
- * when the launch shape or a view's proposal state changes */
+/* views per chunk (1-8, default 8).  Each chunk of views is one chain of
+ * launches (its learned views' proposals, then the four stages, one launch
+ * each); chains are independent and may run concurrently. */
+int vr_batch_set_chunk_views(vr_batch* b, uint32_t views_per_chunk);
+/* replay each frame's launches from a hipGraph (one branch per chain),
+ * built once per in-flight frame slot and updated with each frame's
+ * arguments; rebuilt when the launch shape changes */
 int vr_batch_set_graph(vr_batch* b, int enable);
 /* params[n]: one frame of every view (each view's own render params) */
 int vr_batch_render(vr_batch* b, const vr_render_params* params);

@@ -97,6 +97,7 @@ SIGNATURES = {
     "vr_batch_last_error": (C.c_char_p, [_P]),
     "vr_batch_set_blocks": (C.c_int, [_P, _u32]),
     "vr_batch_set_graph": (C.c_int, [_P, C.c_int]),
+    "vr_batch_set_chunk_views": (C.c_int, [_P, _u32]),
     "vr_batch_render": (C.c_int, [_P, _P]),
     "vr_batch_sync": (C.c_int, [_P]),
     "vr_batch_profile_begin": (C.c_int, [_P]),

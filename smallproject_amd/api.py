@@ -503,6 +503,10 @@ class Batch:
         """Workgroups per stage launch (0: one per tile; else persistent)."""
         self._chk(lib().vr_batch_set_blocks(self.h, blocks), "vr_batch_set_blocks")
 
+    def set_chunk_views(self, n: int):
+        """Views per chunk (1-8); each chunk is an independent chain of launches."""
+        self._chk(lib().vr_batch_set_chunk_views(self.h, n), "vr_batch_set_chunk_views")
+
     def set_graph(self, enable: bool):
         self._chk(lib().vr_batch_set_graph(self.h, int(enable)), "vr_batch_set_graph")
 

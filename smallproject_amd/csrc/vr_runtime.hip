@@ -1108,6 +1108,10 @@ struct vr_batch {
     Slot slot[kSlots];
     std::vector<uint32_t> learned;           // indices of the views with the learned source
     hipEvent_t viewReady = nullptr;          // orders each view's stream before the batch
+    static constexpr int kChainStreams = 4;  // streams of the chains when launched directly
+    hipStream_t chainStream[kChainStreams] = {};
+    hipEvent_t fork = nullptr, join[kChainStreams] = {};
+    uint32_t chunkViews = kChunkViews;       // views per chunk; each chunk is an independent chain
     uint32_t blocks = 0;                     // workgroups per launch (0: one per tile)
     bool graph = false;
     bool anyMis = false;
@@ -1132,30 +1136,58 @@ int bfail(vr_batch* b, int code, const std::string& msg)
         if (e_ != hipSuccess) return bfail(b, VR_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// the node list of a frame: per stage, the proposals before the candidate
-// stage, then one node per chunk
+// The nodes of a frame, chain by chain: a chunk's four stages in order
+// (its learned views' proposals before the candidate stage).  f(k, chain,
+// first, params): node k, its chunk, and whether it starts the chain.
+// Chains share nothing but the device, so they may run concurrently.
 template <class F>
 hipError_t for_each_node(vr_batch* b, vr_batch::Slot& sl, F f)
 {
     const size_t nc = sl.chunks.size();
     sl.chunkArgs.resize(nc * VR_NUM_STAGES);
     size_t k = 0;
-    for (int st = 0; st < VR_NUM_STAGES; st++) {
-        if (st == VR_STAGE_INITIAL)
-            for (ProposalArgs& pa : sl.props) {
-                hipKernelNodeParams p;
-                hipError_t e = proposal_node(&pa, &p);
-                if (e == hipSuccess) e = f(k++, p);
-                if (e != hipSuccess) return e;
-            }
-        for (size_t c = 0; c < nc; c++) {
+    for (size_t c = 0; c < nc; c++) {
+        bool first = true;
+        for (int st = 0; st < VR_NUM_STAGES; st++) {
+            if (st == VR_STAGE_INITIAL)
+                for (size_t li = 0; li < sl.props.size(); li++) {
+                    if (b->learned[li] / b->chunkViews != c) continue;
+                    hipKernelNodeParams p;
+                    hipError_t e = proposal_node(&sl.props[li], &p);
+                    if (e == hipSuccess) e = f(k++, c, first, p);
+                    if (e != hipSuccess) return e;
+                    first = false;
+                }
             hipKernelNodeParams p;
-            hipError_t e = stage_views_node(st, &sl.chunks[c], b->blocks, b->anyMis, &sl.chunkArgs[st * nc + c], &p);
-            if (e == hipSuccess) e = f(k++, p);
+            hipError_t e = stage_views_node(st, &sl.chunks[c], b->blocks, b->anyMis, &sl.chunkArgs[c * VR_NUM_STAGES + st],
+                                            &p);
+            if (e == hipSuccess) e = f(k++, c, first, p);
             if (e != hipSuccess) return e;
+            first = false;
         }
     }
     return hipSuccess;
+}
+
+// chunk tables of every slot for b->chunkViews views per chunk (a view's
+// resolution never changes)
+void build_chunks(vr_batch* b)
+{
+    const uint32_t n = (uint32_t)b->views.size(), cv = b->chunkViews;
+    const uint32_t nc = (n + cv - 1) / cv;
+    for (vr_batch::Slot& sl : b->slot) {
+        sl.chunks.assign(nc, ViewChunk{});
+        for (uint32_t c = 0; c < nc; c++) {
+            ViewChunk& ch = sl.chunks[c];
+            std::memset(&ch, 0, sizeof(ch));
+            ch.n = std::min<uint32_t>(cv, n - c * cv);
+            for (uint32_t j = 0; j < ch.n; j++) {
+                const vr_config& cfg = b->views[c * cv + j]->cfg;
+                ch.start[j + 1] = ch.start[j] + ((cfg.width + 15) / 16) * ((cfg.height + 15) / 16);
+            }
+        }
+        sl.props.assign(b->learned.size(), ProposalArgs{});
+    }
 }
 
 void drop_graph(vr_batch::Slot& sl)
@@ -1187,28 +1219,19 @@ int vr_batch_create(vr_ctx* const* views, uint32_t n, vr_batch** out)
         if (views[i]->cfg.flags & VR_FLAG_MIS_SPATIAL) b->anyMis = true;
         if (views[i]->cfg.source_pdf == VR_SOURCE_LEARNED) b->learned.push_back(i);
     }
-    // chunks of kChunkViews views with their tile tables (fixed: a view's
-    // resolution never changes)
-    const uint32_t nc = (n + kChunkViews - 1) / kChunkViews;
+    build_chunks(b);
     hipError_t e = hipSetDevice(b->device);
-    for (int k = 0; k < vr_batch::kSlots; k++) {
-        vr_batch::Slot& sl = b->slot[k];
-        sl.chunks.assign(nc, ViewChunk{});
-        for (uint32_t c = 0; c < nc; c++) {
-            ViewChunk& ch = sl.chunks[c];
-            std::memset(&ch, 0, sizeof(ch));
-            ch.n = std::min<uint32_t>(kChunkViews, n - c * kChunkViews);
-            for (uint32_t j = 0; j < ch.n; j++) {
-                const vr_config& cfg = views[c * kChunkViews + j]->cfg;
-                ch.start[j + 1] = ch.start[j] + ((cfg.width + 15) / 16) * ((cfg.height + 15) / 16);
-            }
-        }
-        sl.props.assign(b->learned.size(), ProposalArgs{});
+    for (vr_batch::Slot& sl : b->slot) {
         if (e == hipSuccess) e = hipEventCreate(&sl.begin);
         if (e == hipSuccess) e = hipEventCreate(&sl.done);
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->viewReady, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (int k = 0; e == hipSuccess && k < vr_batch::kChainStreams; k++) {
+        e = hipStreamCreateWithFlags(&b->chainStream[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         vr_batch_destroy(b);
         return VR_ERR_HIP;
@@ -1227,6 +1250,11 @@ int vr_batch_destroy(vr_batch* b)
         if (sl.done) (void)hipEventDestroy(sl.done);
     }
     if (b->viewReady) (void)hipEventDestroy(b->viewReady);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (int k = 0; k < vr_batch::kChainStreams; k++) {
+        if (b->chainStream[k]) (void)hipStreamSynchronize(b->chainStream[k]), (void)hipStreamDestroy(b->chainStream[k]);
+        if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return VR_OK;
@@ -1238,6 +1266,20 @@ int vr_batch_set_blocks(vr_batch* b, uint32_t blocks)
 {
     if (!b) return VR_ERR_ARG;
     b->blocks = blocks;
+    return VR_OK;
+}
+
+int vr_batch_set_chunk_views(vr_batch* b, uint32_t views_per_chunk)
+{
+    if (!b || views_per_chunk == 0 || views_per_chunk > (uint32_t)kChunkViews) return VR_ERR_ARG;
+    if (views_per_chunk == b->chunkViews) return VR_OK;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));  // no frame may still read the slots
+    for (vr_batch::Slot& sl : b->slot) {
+        drop_graph(sl);
+        sl.inFlight = false;
+    }
+    b->chunkViews = views_per_chunk;
+    build_chunks(b);
     return VR_OK;
 }
 
@@ -1274,7 +1316,7 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
         c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
         FrameArgs a = frame_args(c, &params[i]);
         a.tileOrder = VR_TILES_LINEAR;
-        sl.chunks[i / kChunkViews].v[i % kChunkViews] = a;
+        sl.chunks[i / b->chunkViews].v[i % b->chunkViews] = a;
         if (c->cfg.source_pdf == VR_SOURCE_LEARNED) {
             ProposalArgs& pa = sl.props[li++];
             pa.a = a, pa.m = mlp_view(c), pa.candIdx = c->candIdx, pa.candPdf = c->candPdf;
@@ -1290,14 +1332,14 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
     }
     BHIPCHK(b, hipEventRecord(sl.begin, b->stream));
     if (b->graph) {
-        const uint64_t key = ((uint64_t)b->blocks << 32) | (b->anyMis ? 1u : 0u);
+        const uint64_t key = ((uint64_t)b->blocks << 32) | ((uint64_t)b->chunkViews << 1) | (b->anyMis ? 1u : 0u);
         if (!sl.exec || sl.key != key) {
             drop_graph(sl);
             BHIPCHK(b, hipGraphCreate(&sl.graph, 0));
-            const hipError_t e = for_each_node(b, sl, [&](size_t k, const hipKernelNodeParams& p) {
+            const hipError_t e = for_each_node(b, sl, [&](size_t k, size_t, bool first, const hipKernelNodeParams& p) {
                 hipGraphNode_t node = nullptr;
-                const hipGraphNode_t* dep = k ? &sl.nodes[k - 1] : nullptr;
-                const hipError_t r = hipGraphAddKernelNode(&node, sl.graph, dep, k ? 1 : 0, &p);
+                const hipGraphNode_t* dep = first ? nullptr : &sl.nodes[k - 1];
+                const hipError_t r = hipGraphAddKernelNode(&node, sl.graph, dep, first ? 0 : 1, &p);
                 if (r == hipSuccess) sl.nodes.push_back(node);
                 return r;
             });
@@ -1315,15 +1357,29 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
         } else {
             // this frame's arguments into the slot's nodes (the slot's
             // previous launch has finished)
-            BHIPCHK(b, for_each_node(b, sl, [&](size_t k, const hipKernelNodeParams& p) {
+            BHIPCHK(b, for_each_node(b, sl, [&](size_t k, size_t, bool, const hipKernelNodeParams& p) {
                         return hipGraphExecKernelNodeSetParams(sl.exec, sl.nodes[k], &p);
                     }));
         }
         BHIPCHK(b, hipGraphLaunch(sl.exec, b->stream));
     } else {
-        BHIPCHK(b, for_each_node(b, sl, [&](size_t, const hipKernelNodeParams& p) {
-                    return hipLaunchKernel(p.func, p.gridDim, p.blockDim, p.kernelParams, 0, b->stream);
+        // chains on their own streams (round robin), forked from and joined
+        // back into the batch stream
+        const size_t nc = sl.chunks.size();
+        const int ns = (int)std::min<size_t>(nc, vr_batch::kChainStreams);
+        if (ns > 1) {
+            BHIPCHK(b, hipEventRecord(b->fork, b->stream));
+            for (int k = 0; k < ns; k++) BHIPCHK(b, hipStreamWaitEvent(b->chainStream[k], b->fork, 0));
+        }
+        BHIPCHK(b, for_each_node(b, sl, [&](size_t, size_t c, bool, const hipKernelNodeParams& p) {
+                    hipStream_t st = ns > 1 ? b->chainStream[c % vr_batch::kChainStreams] : b->stream;
+                    return hipLaunchKernel(p.func, p.gridDim, p.blockDim, p.kernelParams, 0, st);
                 }));
+        if (ns > 1)
+            for (int k = 0; k < ns; k++) {
+                BHIPCHK(b, hipEventRecord(b->join[k], b->chainStream[k]));
+                BHIPCHK(b, hipStreamWaitEvent(b->stream, b->join[k], 0));
+            }
     }
     BHIPCHK(b, hipEventRecord(sl.done, b->stream));
     sl.inFlight = true;

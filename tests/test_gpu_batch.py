@@ -59,8 +59,9 @@ def _render_single(scs, frames):
     return out
 
 
-@pytest.mark.parametrize("graph,blocks", [(False, 0), (True, 0), (True, 7)])
-def test_batch_renders_each_views_own_frames(gpu, graph, blocks):
+@pytest.mark.parametrize("graph,blocks,chunk", [(False, 0, 8), (True, 0, 8), (True, 7, 8), (False, 0, 1),
+                                                (True, 0, 2)])
+def test_batch_renders_each_views_own_frames(gpu, graph, blocks, chunk):
     scs = _instances()
     frames = 3
     want = _render_single(scs, frames)
@@ -68,6 +69,7 @@ def test_batch_renders_each_views_own_frames(gpu, graph, blocks):
     b = Batch(worlds)
     b.set_graph(graph)
     b.set_blocks(blocks)
+    b.set_chunk_views(chunk)
     for f in range(frames):
         for sc, w in zip(scs, worlds):
             pos, tgt = sc.camera(f)
