@@ -135,19 +135,45 @@ __device__ __forceinline__ void softmax_stats(const MlpView& m, const uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
-// inference path (k_proposal, k_mlp_probs): one padded activation buffer per
-// wave (H1 is overwritten by H2, rows padded to 72 so the 16-byte operand
+// inference path (k_proposal, k_mlp_probs): inputs in registers, one padded
+// activation buffer per wave (H1 is overwritten by H2, rows padded to 72 so the 16-byte operand
 // loads of neighbouring rows fall in different LDS banks) and the output
 // bias staged in LDS with -inf for clusters without lights, used as the MFMA
 // accumulator's initial value
 // ---------------------------------------------------------------------------
-constexpr int kLdX = 24;  // padded row strides (bf16 elements, 16-byte multiples)
-constexpr int kLdH = 72;
+constexpr int kLdH = 72;  // padded row stride (bf16 elements, a 16-byte multiple)
 
 struct __attribute__((aligned(16))) InferLds {
-    __bf16 X[64 * kLdX];  // inputs, sample-major
     __bf16 H[64 * kLdH];  // relu(layer 1), then relu(layer 2)
 };
+
+// A wave's 64 inputs stay in registers: lane p holds sample p's 16 padded
+// inputs as two halves (k 0-7, 8-15).  The layer-1 MFMA of row block mb
+// needs, in lane l, sample mb*32 + l%32's half l/32: the lane's own half or
+// the other half of lane l^32.
+struct InputRegs {
+    bf16x8 lo, hi;
+};
+
+__device__ __forceinline__ bf16x8 shfl_xor32(bf16x8 v)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 u;
+    __builtin_memcpy(&u, &v, 16);
+#pragma unroll
+    for (int i = 0; i < 4; i++) u[i] = (uint32_t)__shfl_xor((int)u[i], 32);
+    bf16x8 o;
+    __builtin_memcpy(&o, &u, 16);
+    return o;
+}
+
+__device__ __forceinline__ InputRegs pack_inputs(const float (&in)[16])
+{
+    InputRegs r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.lo[i] = to_bf16(in[i]), r.hi[i] = to_bf16(in[8 + i]);
+    return r;
+}
 
 // masked output bias: b3[c], or -inf for an empty or absent cluster
 __device__ __forceinline__ void stage_bias(const MlpView& m, float* sBias)
@@ -156,11 +182,16 @@ __device__ __forceinline__ void stage_bias(const MlpView& m, float* sBias)
         sBias[c] = (c < (int)m.nclusters && m.clusterStart[c + 1] > m.clusterStart[c]) ? m.b3[c] : -INFINITY;
 }
 
-__device__ void forward_infer(const MlpView& m, InferLds& s)
+__device__ void forward_infer(const MlpView& m, InferLds& s, const InputRegs& x)
 {
+    const int l = threadIdx.x & 63;
+    const bf16x8 olo = shfl_xor32(x.lo), ohi = shfl_xor32(x.hi);
+    const bf16x8 a0 = l < 32 ? x.lo : ohi;  // row block 0: samples 0-31
+    const bf16x8 a1 = l < 32 ? olo : x.hi;  // row block 1: samples 32-63
     for (int mb = 0; mb < 2; mb++)
         for (int nb = 0; nb < 2; nb++) {
-            f32x16 acc = mfma_nt(s.X + mb * 32 * kLdX, kLdX, m.W1b + nb * 32 * 16, 16, 16, zero16());
+            const bf16x8 bv = *reinterpret_cast<const bf16x8*>(m.W1b + nb * 32 * 16 + (l & 31) * 16 + ((l >> 5) << 3));
+            const f32x16 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(mb ? a1 : a0, bv, zero16(), 0, 0, 0);
             const int j = nb * 32 + acc_col();
             const float bj = m.b1[j];
 #pragma unroll
@@ -253,6 +284,7 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
         x = tx * 16 + (wv & 1) * 8 + (p & 7);
         y = ty * 16 + (wv >> 1) * 8 + (p >> 3);
     };
+    InputRegs xin;
     {
         int x, y;
         pixel_of(lane, x, y);
@@ -271,11 +303,10 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
                 in[3 + mlp_face(hitw)] = 1.0f;
             }
         }
-#pragma unroll
-        for (int i = 0; i < 16; i++) s.X[lane * kLdX + i] = to_bf16(in[i]);
+        xin = pack_inputs(in);
     }
-    __syncthreads();
-    forward_infer(m, s);
+    __syncthreads();  // the staged bias and powers
+    forward_infer(m, s, xin);
     const float keep = 1.0f - m.mix;
     const size_t plane = (size_t)a.W * a.H;
     for (int nb = 0; nb < 2; nb++) {
@@ -286,8 +317,9 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
         const size_t idx = valid ? (size_t)y * a.W + x : 0;
         float M, S, Slo;
         softmax_infer(m, sBias, s, nb, M, S, Slo);
+        const float kS = keep / S;  // keep * q_c = exp(z_c - M) * kS
         // mixture mass held by the lower lane
-        const float massLo = keep * (Slo / S) + m.mix * m.powerLow;
+        const float massLo = Slo * kS + m.mix * m.powerLow;
         // target jitters from the pixel's proposal stream (the shading stream
         // stays untouched); both lanes of the pair see the same u_j
         const uint32_t ubase = wang_hash((valid ? a.cur.seed[idx] : 0u) ^ salt);
@@ -313,7 +345,7 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const uint32_t c = (uint32_t)(mb * 32 + acc_row(q));
-                const float mc = keep * (__expf(z[q] - M) / S) + m.mix * sPower[c];
+                const float mc = __expf(z[q] - M) * kS + m.mix * sPower[c];
                 cdf = cdf + mc;
                 if (mc > 0.0f) {
                     while (tk < cdf) {
@@ -361,13 +393,16 @@ __global__ __launch_bounds__(64) void k_mlp_probs(MlpView m, const float* __rest
     const int lane = threadIdx.x & 63;
     const uint32_t base = blockIdx.x * 64;
     stage_bias(m, sBias);
+    InputRegs xin;
     {
         const uint32_t b = base + lane;
+        float in[16];
 #pragma unroll
-        for (int i = 0; i < 16; i++) s.X[lane * kLdX + i] = to_bf16(b < B && i < 9 ? x[(size_t)b * 9 + i] : 0.0f);
+        for (int i = 0; i < 16; i++) in[i] = b < B && i < 9 ? x[(size_t)b * 9 + i] : 0.0f;
+        xin = pack_inputs(in);
     }
     __syncthreads();
-    forward_infer(m, s);
+    forward_infer(m, s, xin);
     for (int nb = 0; nb < 2; nb++) {
         float M, S, Slo;
         softmax_infer(m, sBias, s, nb, M, S, Slo);

@@ -181,8 +181,9 @@ def dynamic(name: str = "c5_dynamic_1024", width=None, height=None, frames: int 
     MSE x frame time is worse than the plain method's, the learned context
     falls back to the plain source (mixture weight 1, which samples exactly
     as the power source) until the next check, else it uses the proposal.
-    The network is first trained along `pretrain_frames` frames of the path
-    on a separate plain context."""
+    The learned context's time per frame includes its share of the
+    fine-tuning (wall time).  The network is first trained along
+    `pretrain_frames` frames of the path on a separate plain context."""
     base = CONFIGS[name]
     scP = dataclasses.replace(base, source=_lib.SOURCE_POWER, quirks=EVAL_QUIRKS)
     scL = dataclasses.replace(base, source=_lib.SOURCE_LEARNED, quirks=EVAL_QUIRKS)
@@ -228,7 +229,10 @@ def dynamic(name: str = "c5_dynamic_1024", width=None, height=None, frames: int 
             else:
                 msL, nL = msL + float(s_.sum()), nL + n_
         if f % finetune_every == finetune_every - 1:
+            # the fine-tuning's wall time counts toward the learned context's frames
+            t1 = time.perf_counter()
             rL.mlp_train(nsamples, lr, train_steps, source=rP)
+            msL += 1000.0 * (time.perf_counter() - t1)
         if f % check_every == check_every - 1:
             wP.look_at(pos, tgt)
             p = wP.prepare_frame()
