@@ -258,18 +258,29 @@ __device__ uint32_t trace(const WorldView& w, f3 A, f3 B, int steps, uint32_t th
                     tmz = ((float)((p & 1023u) + oz) - A.z) * rV.z;
                     p &= 7u + (7u << 10) + (7u << 20);
                     const uint32_t* __restrict__ brick = w.bricks + (size_t)(o >> 1) * 512u;
-                    // at most 22 steps cross a brick; the reference unrolls 24
+                    // at most 22 steps cross a brick; the reference unrolls 24.
+                    // Software-pipelined: the next voxel's position does not
+                    // depend on the current voxel, so its load is issued
+                    // before the current one is tested and two loads are in
+                    // flight per lane.  The state on return and on leaving
+                    // the loop is the plain loop's.
+                    uint32_t vv = brick[(p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
                     for (int k = 0; k < 24; k++) {
-                        const uint32_t vv = brick[(p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
+                        const float tn = fminf(tmx, fminf(tmy, tmz));
+                        float nx = tmx, ny = tmy, nz = tmz;
+                        uint32_t pn = p, ln = 0;
+                        if (tn == tmx) nx += td.x, pn += dx;
+                        if (tn == tmy) ny += td.y, pn += dy, ln = 1;
+                        if (tn == tmz) nz += td.z, pn += dz, ln = 2;
+                        const bool more = !(pn & kTopMask3) && k + 1 < 24;
+                        const uint32_t vn = more ? brick[(pn >> 20) + ((pn >> 7) & 56u) + ((pn & 7u) << 6)] : 0u;
                         if (voxel_stops<KIND>(vv, through)) {
                             dist = t + to, side = last;
                             return vv;
                         }
-                        t = fminf(tmx, fminf(tmy, tmz)), last = 0;
-                        if (t == tmx) tmx += td.x, p += dx;
-                        if (t == tmy) tmy += td.y, p += dy, last = 1;
-                        if (t == tmz) tmz += td.z, p += dz, last = 2;
-                        if (p & kTopMask3) break;
+                        t = tn, last = ln, tmx = nx, tmy = ny, tmz = nz, p = pn;
+                        if (!more) break;
+                        vv = vn;
                     }
                     tmx = gx, tmy = gy, tmz = gz;
                 }
