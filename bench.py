@@ -295,11 +295,15 @@ def main() -> int:
         step(f)
     r.sync()
     log(f"bench: {args.warmup} warm-up frames done")
-    from smallproject_amd.autotune import ORDERS, tune_tile_order
+    from smallproject_amd.autotune import ORDERS, tune_tile_order, tune_waves
+    waves, waves_ms = None, None
     if args.tile_order == "auto":
         tile_order, tune_ms = tune_tile_order(r, lambda i: step(args.warmup + i))
         log("bench: tile orders " + "; ".join(f"{k} " + " ".join(f"{v:.4f}" for v in ms) for k, ms in tune_ms.items())
             + f" -> {tile_order}")
+        waves, waves_ms = tune_waves(r, lambda i: step(args.warmup + i))
+        log("bench: waves per SIMD " + "; ".join(f"{k} " + " ".join(f"{v:.4f}" for v in ms)
+                                                for k, ms in waves_ms.items()) + f" -> {waves}")
     else:
         tile_order, tune_ms = args.tile_order, None
         r.set_tile_order(ORDERS[tile_order])
@@ -343,7 +347,10 @@ def main() -> int:
                    "resolution": [sc.width, sc.height], "frames_in_path": sc.frames,
                    "tile_order": tile_order,
                    "tile_order_stage_ms": None if tune_ms is None else
-                   {k: [round(v, 4) for v in ms] for k, ms in tune_ms.items()}},
+                   {k: [round(v, 4) for v in ms] for k, ms in tune_ms.items()},
+                   "waves_per_simd": waves,
+                   "waves_stage_ms": None if waves_ms is None else
+                   {("default" if k == 0 else str(k)): [round(v, 4) for v in ms] for k, ms in waves_ms.items()}},
         "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -220,6 +220,12 @@ int vr_sync(vr_ctx* ctx);
  * reference; outputs are identical for every order). */
 int vr_set_tile_order(vr_ctx* ctx, int32_t stage, uint32_t order);
 
+/* Occupancy target (waves per SIMD, 5-8; 0 = the stage's default) of stage
+ * VR_STAGE_* (-1: every stage) for the following frames: a performance
+ * setting with no counterpart in the reference (outputs are identical).
+ * More waves hide more of the DDA's load latency; fewer spill less. */
+int vr_set_stage_waves(vr_ctx* ctx, int32_t stage, uint32_t waves);
+
 /* Read-backs (blocking), n = width*height.
  * hits (current frame): hitword = voxel | side<<24 | (D[side] > 0)<<26,
  * dist, seed.  Any output pointer may be NULL. */

@@ -70,6 +70,7 @@ SIGNATURES = {
     "vr_render_frame": (C.c_int, [_P, _P]),
     "vr_sync": (C.c_int, [_P]),
     "vr_set_tile_order": (C.c_int, [_P, _i32, _u32]),
+    "vr_set_stage_waves": (C.c_int, [_P, _i32, _u32]),
     "vr_read_hits": (C.c_int, [_P, _P, _P, _P]),
     "vr_read_reservoirs": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "vr_read_frame": (C.c_int, [_P, _P]),

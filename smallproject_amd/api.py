@@ -86,6 +86,10 @@ class Renderer:
     def sync(self):
         self._chk(lib().vr_sync(self.h), "vr_sync")
 
+    def set_stage_waves(self, waves: int, stage: int = -1):
+        """Occupancy target (5-8 waves per SIMD, 0 = default) of a stage (-1: all)."""
+        self._chk(lib().vr_set_stage_waves(self.h, stage, waves), "vr_set_stage_waves")
+
     def set_tile_order(self, order: int, stage: int = -1):
         self._chk(lib().vr_set_tile_order(self.h, stage, order), "vr_set_tile_order")
 

@@ -1,5 +1,6 @@
 """Launch-shape autotuner: picks the tile order (include/vr_api.h,
-VR_TILES_*) of every stage by timing the caller's own frames.
+VR_TILES_*) and the occupancy target (waves per SIMD, vr_set_stage_waves)
+of every stage by timing the caller's own frames.
 
 Every order renders identical pixels, so the search runs on live frames.
 Each order renders the same `frames` frames of the caller's sequence (frame
@@ -8,7 +9,8 @@ same frames) with the per-stage event profiler on; each stage keeps the
 order under which it ran fastest.  The comparison is repeated `rounds` times
 in alternating order and each order's best round counts (robust to a one-off
 clock or queue hiccup).  An order must beat the linear order by more than
-`margin` to be chosen.
+`margin` to be chosen.  The occupancy targets are then searched the same
+way with the chosen orders, against each stage's default target.
 """
 from __future__ import annotations
 
@@ -42,3 +44,33 @@ def tune_tile_order(renderer, render_frame, frames: int = 8, rounds: int = 2, ma
         renderer.set_tile_order(ORDERS[best], st)
         chosen[_lib.STAGE_NAMES[st]] = best
     return chosen, times
+
+
+WAVES = (0, 5, 6, 7, 8)  # 0: the stage's default target
+
+
+def tune_waves(renderer, render_frame, frames: int = 8, rounds: int = 2, margin: float = 0.02):
+    """Occupancy target per stage, as tune_tile_order picks orders (on the
+    orders already set).  Returns ({stage name: waves}, {waves: [ms per
+    stage]}) and leaves the chosen targets set."""
+    nst = _lib.NUM_STAGES
+    times = {wv: [float("inf")] * nst for wv in WAVES}
+    for rd in range(rounds):
+        for wv in (WAVES if rd % 2 == 0 else WAVES[::-1]):
+            renderer.set_stage_waves(wv)
+            render_frame(0)
+            renderer.sync()
+            renderer.profile_begin()
+            for i in range(frames):
+                render_frame(1 + i)
+            s, n = renderer.profile_read()
+            times[wv] = [min(t, float(v) / max(n, 1)) for t, v in zip(times[wv], s)]
+    chosen = {}
+    for st in range(nst):
+        best = min(WAVES, key=lambda wv: times[wv][st])
+        if times[best][st] > (1.0 - margin) * times[0][st]:
+            best = 0
+        renderer.set_stage_waves(best, st)
+        chosen[_lib.STAGE_NAMES[st]] = best
+    return chosen, times
+

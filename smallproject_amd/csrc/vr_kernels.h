@@ -38,6 +38,7 @@ struct FrameArgs {
     int32_t shadowSteps;
     uint32_t tileOrder;  // VR_TILES_*
     uint32_t misSpatial; // spatial merge with MIS weights (VR_FLAG_MIS_SPATIAL)
+    uint32_t waves;      // occupancy target of this stage's kernel (0: kDefaultWaves)
     // learned source: candidate slot j of pixel i at [j * W*H + i] holds a
     // cluster and its mixture mass; inside the cluster the light is drawn
     // with the cluster's power alias table
@@ -78,6 +79,9 @@ __device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
     ty = (int)(t / nbx);
 }
 
+// default occupancy targets (waves per SIMD) of the stage kernels: primary,
+// candidates, spatial, shading
+constexpr uint32_t kDefaultWaves[4] = {8, 7, 8, 8};
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s);
 // Up to kChunkViews views rendered by one launch per stage: their
 // FrameArgs travel in the kernel argument (8 x 448 B + the tile table,

@@ -630,20 +630,26 @@ __device__ __forceinline__ void many_views(const ViewChunk& c, F f)
 // shading 0.58 -> 0.51 ms at 8, the MIS spatial merge 3.17 -> 2.59 ms at 8).
 #define VR_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
-__global__ __launch_bounds__(256) VR_WAVES(8) void k_primary(FrameArgs a)
+// One-view stage kernels, instantiated for occupancy targets of 5-8 waves
+// per SIMD (FrameArgs::waves, picked per stage by vr_set_stage_waves or the
+// autotuner; the defaults are the measured best above).
+template <int W>
+__global__ __launch_bounds__(256) VR_WAVES(W) void k_primary(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { primary_px(v, x, y); });
 }
-__global__ __launch_bounds__(256) VR_WAVES(7) void k_initial(FrameArgs a)
+template <int W>
+__global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
 }
-template <bool kMis>
-__global__ __launch_bounds__(256) VR_WAVES(8) void k_spatial(FrameArgs a)
+template <bool kMis, int W>
+__global__ __launch_bounds__(256) VR_WAVES(W) void k_spatial(FrameArgs a)
 {
     one_view<kMis>(a, [](const FrameArgs& v, int x, int y) { spatial_px<kMis>(v, x, y); });
 }
-__global__ __launch_bounds__(256) VR_WAVES(8) void k_shade(FrameArgs a)
+template <int W>
+__global__ __launch_bounds__(256) VR_WAVES(W) void k_shade(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { shade_px(v, x, y); });
 }
@@ -793,20 +799,38 @@ __global__ __launch_bounds__(256) void k_ground_truth(FrameArgs a, uint32_t spp,
 // ---------------------------------------------------------------------------
 static inline dim3 frame_grid(uint32_t W, uint32_t H) { return dim3((W + 15) / 16, (H + 15) / 16); }
 
+template <int W>
+static const void* stage_kernel_w(int stage, bool mis)
+{
+    switch (stage) {
+    case 0: return reinterpret_cast<const void*>(&k_primary<W>);
+    case 1: return reinterpret_cast<const void*>(&k_initial<W>);
+    case 2: return mis ? reinterpret_cast<const void*>(&k_spatial<true, W>)
+                       : reinterpret_cast<const void*>(&k_spatial<false, W>);
+    case 3: return reinterpret_cast<const void*>(&k_shade<W>);
+    default: return nullptr;
+    }
+}
+
+// waves 0 = the stage's default target
+static const void* stage_kernel(int stage, bool mis, uint32_t waves)
+{
+    if (waves == 0) waves = kDefaultWaves[stage < 0 || stage > 3 ? 0 : stage];
+    switch (waves) {
+    case 5: return stage_kernel_w<5>(stage, mis);
+    case 6: return stage_kernel_w<6>(stage, mis);
+    case 7: return stage_kernel_w<7>(stage, mis);
+    case 8: return stage_kernel_w<8>(stage, mis);
+    default: return nullptr;
+    }
+}
+
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s)
 {
-    const dim3 g = frame_grid(a.W, a.H);
-    switch (stage) {
-    case 0: hipLaunchKernelGGL(k_primary, g, dim3(256), 0, s, a); break;
-    case 1: hipLaunchKernelGGL(k_initial, g, dim3(256), 0, s, a); break;
-    case 2:
-        if (a.misSpatial) hipLaunchKernelGGL(k_spatial<true>, g, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_spatial<false>, g, dim3(256), 0, s, a);
-        break;
-    case 3: hipLaunchKernelGGL(k_shade, g, dim3(256), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const void* k = stage_kernel(stage, a.misSpatial != 0u, a.waves);
+    if (!k) return hipErrorInvalidValue;
+    void* args[] = {const_cast<FrameArgs*>(&a)};
+    return hipLaunchKernel(k, frame_grid(a.W, a.H), dim3(256), args, 0, s);
 }
 
 template <bool kP>

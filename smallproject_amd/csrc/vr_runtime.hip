@@ -93,6 +93,7 @@ struct vr_ctx {
     float* chunkLoss = nullptr;
     uint32_t trainCap = 0;
     Camera lastCam{};             // camera of the last rendered frame
+    uint32_t stageWaves[VR_NUM_STAGES] = {};  // occupancy target per stage (0: default)
     // scene motion on the device (allocated by vr_set_light_motion)
     float4* lightPos = nullptr;
     float4* lightVel = nullptr;
@@ -601,6 +602,7 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
     for (int s = 0; s < VR_NUM_STAGES; s++) {
         FrameArgs as = a;
         as.tileOrder = (c->cfg.flags >> (2 * s)) & 3u;
+        as.waves = c->stageWaves[s];
         // the proposal runs after the primary hits and is timed with the
         // candidate stage
         if (s == VR_STAGE_INITIAL && learned)
@@ -621,6 +623,14 @@ int vr_sync(vr_ctx* c)
 {
     if (!c) return VR_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VR_OK;
+}
+
+int vr_set_stage_waves(vr_ctx* c, int32_t stage, uint32_t waves)
+{
+    if (!c || stage < -1 || stage >= VR_NUM_STAGES || (waves != 0 && (waves < 5 || waves > 8))) return VR_ERR_ARG;
+    for (int s = 0; s < VR_NUM_STAGES; s++)
+        if (stage < 0 || stage == s) c->stageWaves[s] = waves;
     return VR_OK;
 }
 

@@ -138,3 +138,30 @@ def test_tile_orders_render_identical_frames(gpu, size):
     for o in outs[1:]:
         for k, (x, y) in enumerate(zip(outs[0], o)):
             assert np.array_equal(x, y), f"output {k} differs between tile orders"
+
+
+def test_occupancy_targets_render_identical_frames(gpu):
+    """Every occupancy-target instantiation of the stage kernels computes the
+    same bits (area lights with the MIS merge, so every stage traces)."""
+    import dataclasses
+    sc = dataclasses.replace(configs.CONFIGS["c1_point_128"], point=False, flags=_lib.FLAG_MIS_SPATIAL)
+    outs = []
+    for waves in (0, 5, 6, 7, 8):
+        w = configs.build_world(sc, 160, 96)
+        w.force_sync()
+        r = w.renderer
+        r.set_stage_waves(waves)
+        for f in range(3):
+            pos, tgt = sc.camera(f)
+            w.look_at(pos, tgt)
+            w.render()
+        hw, d, s = r.hits()
+        a, b, m = r.reservoirs(0)
+        outs.append((hw, d.view(np.uint32), s, a.view(np.uint32), b.view(np.uint32), m, r.frame().view(np.uint32)))
+        w.close()
+    for o in outs[1:]:
+        for k, (x, y) in enumerate(zip(outs[0], o)):
+            assert np.array_equal(x, y), f"output {k} differs between occupancy targets"
+    with pytest.raises(_lib.VrError):
+        r2 = configs.build_world(sc, 32, 32).renderer
+        r2.set_stage_waves(9)
