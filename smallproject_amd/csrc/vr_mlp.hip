@@ -435,134 +435,182 @@ struct __attribute__((aligned(16))) GradLds {
     __bf16 dH1T[64 * 64];
 };
 
-__global__ __launch_bounds__(64) void k_mlp_grad(MlpView m, const float* __restrict__ x, const uint32_t* __restrict__ tgt,
-                                                 const float* __restrict__ wgt, uint32_t B, float invB,
-                                                 float* __restrict__ partial, float* __restrict__ lossOut)
+// Four waves per 64-sample chunk share its LDS: every matrix product of the
+// backward pass is split into 32x32 output tiles dealt to the waves, with
+// a barrier between dependent products.  The softmax statistics of a
+// sample combine the two waves that hold its two halves of the clusters.
+__global__ __launch_bounds__(256) void k_mlp_grad(MlpView m, const float* __restrict__ x, const uint32_t* __restrict__ tgt,
+                                                  const float* __restrict__ wgt, uint32_t B, float invB,
+                                                  float* __restrict__ partial, float* __restrict__ lossOut)
 {
     __shared__ GradLds s;
-    const int lane = threadIdx.x & 63;
+    __shared__ float stats[4][2][64];  // per wave: max and sum of exp of its cluster half, per sample
+    __shared__ float lossW[4];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const uint32_t base = blockIdx.x * 64;
     float* g = partial + (size_t)blockIdx.x * kMlpParams;
     {
-        const uint32_t b = base + lane;
-        for (int i = 0; i < 16; i++) {
+        // inputs: sample tid & 63, features 4 * wv .. +3 (9 real, zero-padded to 16)
+        const int smp = tid & 63;
+        const uint32_t b = base + smp;
+        for (int i = wv * 4; i < wv * 4 + 4; i++) {
             const __bf16 v = to_bf16(b < B && i < 9 ? x[(size_t)b * 9 + i] : 0.0f);
-            s.f.X[lane * 16 + i] = v;
-            s.XT[i * 64 + lane] = v;
-            s.XT[(16 + i) * 64 + lane] = to_bf16(0.0f);
+            s.f.X[smp * 16 + i] = v;
+            s.XT[i * 64 + smp] = v;
+            s.XT[(16 + i) * 64 + smp] = to_bf16(0.0f);
         }
     }
     __syncthreads();
-    forward_hidden(m, s.f);
-    for (int i = 0; i < 64; i++) {
+    const int tm = wv >> 1, tn = wv & 1;  // this wave's tile of a 64x64 product
+    {
+        const f32x16 acc = mfma_nt(s.f.X + tm * 32 * 16, 16, m.W1b + tn * 32 * 16, 16, 16, zero16());
+        const int j = tn * 32 + acc_col();
+        const float bj = m.b1[j];
+#pragma unroll
+        for (int q = 0; q < 16; q++) s.f.H1[(tm * 32 + acc_row(q)) * 64 + j] = to_bf16(fmaxf(acc[q] + bj, 0.0f));
+    }
+    __syncthreads();
+    {
+        const f32x16 acc = mfma_nt(s.f.H1 + tm * 32 * 64, 64, m.W2b + tn * 32 * 64, 64, 64, zero16());
+        const int j = tn * 32 + acc_col();
+        const float bj = m.b2[j];
+#pragma unroll
+        for (int q = 0; q < 16; q++) s.f.H2[(tm * 32 + acc_row(q)) * 64 + j] = to_bf16(fmaxf(acc[q] + bj, 0.0f));
+    }
+    __syncthreads();
+    // feature-major copies (wave wv: features 16 wv .. +15)
+    for (int i = wv * 16; i < wv * 16 + 16; i++) {
         s.H1T[i * 64 + lane] = s.f.H1[lane * 64 + i];
         s.H2T[i * 64 + lane] = s.f.H2[lane * 64 + i];
     }
-    float loss = 0.0f;
-    for (int nb = 0; nb < 2; nb++) {
-        float M, S, Slo;
-        softmax_stats(m, m.clusterStart, s.f, nb, M, S, Slo);
-        const uint32_t b = base + nb * 32 + (lane & 31);
+    // logits: wave wv holds sample block nb = wv >> 1, cluster blocks
+    // 4 (wv & 1) .. +3; statistics over its half, then over both halves
+    const int nb = wv >> 1, mb0 = (wv & 1) * 4;
+    f32x16 z[4];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        z[k] = logits_block(m, m.clusterStart, s.f, mb0 + k, nb);
+#pragma unroll
+        for (int q = 0; q < 16; q++) mx = fmaxf(mx, z[k][q]);
+    }
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    float sum = 0.0f;
+    if (mx != -INFINITY) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) sum += z[k][q] == -INFINITY ? 0.0f : __expf(z[k][q] - mx);
+    }
+    sum += __shfl_xor(sum, 32);
+    if (lane < 32) stats[wv][0][lane] = mx, stats[wv][1][lane] = sum;
+    __syncthreads();
+    {
+        const int other = wv ^ 1, col = lane & 31;
+        const float omx = stats[other][0][col], osum = stats[other][1][col];
+        const float M = fmaxf(mx, omx);
+        const float S = (mx == -INFINITY ? 0.0f : sum * __expf(mx - M)) + (omx == -INFINITY ? 0.0f : osum * __expf(omx - M));
+        const int smp = nb * 32 + col;
+        const uint32_t b = base + smp;
         const float w = b < B ? wgt[b] * invB : 0.0f;
         const uint32_t t = b < B ? tgt[b] : 0u;
-        for (int mb = 0; mb < 8; mb++) {
-            const f32x16 z = logits_block(m, m.clusterStart, s.f, mb, nb);
+        float loss = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
 #pragma unroll
             for (int q = 0; q < 16; q++) {
-                const int c = mb * 32 + acc_row(q);
-                const float p = z[q] == -INFINITY ? 0.0f : __expf(z[q] - M) / S;
+                const int c = (mb0 + k) * 32 + acc_row(q);
+                const float p = z[k][q] == -INFINITY ? 0.0f : __expf(z[k][q] - M) / S;
                 const float d = w * (p - (c == (int)t ? 1.0f : 0.0f));
-                const int smp = nb * 32 + (lane & 31);
                 const __bf16 db = to_bf16(d);
                 s.dZT[c * 64 + smp] = db;
                 s.dZ[smp * 256 + c] = db;
-                if (c == (int)t && w > 0.0f) loss += -w * (z[q] - M - __logf(S));
+                if (c == (int)t && w > 0.0f) loss += -w * (z[k][q] - M - __logf(S));
             }
-        }
+        for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+        if (lane == 0) lossW[wv] = loss;
     }
     __syncthreads();
-    // dW3 = dZ^T H2 and db3
-    for (int mb = 0; mb < 8; mb++)
-        for (int nb = 0; nb < 2; nb++) {
-            const f32x16 acc = mfma_nt(s.dZT + mb * 32 * 64, 64, s.H2T + nb * 32 * 64, 64, 64, zero16());
+    // dW3 = dZ^T H2 (16 tiles, 4 per wave) and db3 (one cluster per thread)
+    for (int k = wv * 4; k < wv * 4 + 4; k++) {
+        const int mb = k >> 1, nbw = k & 1;
+        const f32x16 acc = mfma_nt(s.dZT + mb * 32 * 64, 64, s.H2T + nbw * 32 * 64, 64, 64, zero16());
 #pragma unroll
-            for (int q = 0; q < 16; q++)
-                g[kOffW3 + (mb * 32 + acc_row(q)) * 64 + nb * 32 + acc_col()] = acc[q];
-        }
-    for (int c = lane; c < 256; c += 64) {
-        float sum = 0.0f;
-        for (int i = 0; i < 64; i++) sum += from_bf16(s.dZT[c * 64 + i]);
-        g[kOffB3 + c] = sum;
+        for (int q = 0; q < 16; q++) g[kOffW3 + (mb * 32 + acc_row(q)) * 64 + nbw * 32 + acc_col()] = acc[q];
+    }
+    {
+        float sb = 0.0f;
+        for (int i = 0; i < 64; i++) sb += from_bf16(s.dZT[tid * 64 + i]);
+        g[kOffB3 + tid] = sb;
     }
     // dH2 = dZ W3, masked by relu'(layer 2)
-    for (int mb = 0; mb < 2; mb++)
-        for (int nb = 0; nb < 2; nb++) {
-            const f32x16 acc = mfma_nt(s.dZ + mb * 32 * 256, 256, m.W3Tb + nb * 32 * 256, 256, 256, zero16());
-            const int h = nb * 32 + acc_col();
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int smp = mb * 32 + acc_row(q);
-                const float d = from_bf16(s.f.H2[smp * 64 + h]) > 0.0f ? acc[q] : 0.0f;
-                const __bf16 db = to_bf16(d);
-                s.dH2[smp * 64 + h] = db;
-                s.dH2T[h * 64 + smp] = db;
-            }
-        }
-    __syncthreads();
-    // dW2 = dH2^T H1 and db2
-    for (int mb = 0; mb < 2; mb++)
-        for (int nb = 0; nb < 2; nb++) {
-            const f32x16 acc = mfma_nt(s.dH2T + mb * 32 * 64, 64, s.H1T + nb * 32 * 64, 64, 64, zero16());
-#pragma unroll
-            for (int q = 0; q < 16; q++)
-                g[kOffW2 + (mb * 32 + acc_row(q)) * 64 + nb * 32 + acc_col()] = acc[q];
-        }
     {
-        float sum = 0.0f;
-        for (int i = 0; i < 64; i++) sum += from_bf16(s.dH2T[lane * 64 + i]);
-        g[kOffB2 + lane] = sum;
-    }
-    // dH1 = dH2 W2, masked by relu'(layer 1)
-    for (int mb = 0; mb < 2; mb++)
-        for (int nb = 0; nb < 2; nb++) {
-            const f32x16 acc = mfma_nt(s.dH2 + mb * 32 * 64, 64, m.W2Tb + nb * 32 * 64, 64, 64, zero16());
-            const int h = nb * 32 + acc_col();
+        const f32x16 acc = mfma_nt(s.dZ + tm * 32 * 256, 256, m.W3Tb + tn * 32 * 256, 256, 256, zero16());
+        const int h = tn * 32 + acc_col();
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int smp = mb * 32 + acc_row(q);
-                const float d = from_bf16(s.f.H1[smp * 64 + h]) > 0.0f ? acc[q] : 0.0f;
-                s.dH1T[h * 64 + smp] = to_bf16(d);
-            }
+        for (int q = 0; q < 16; q++) {
+            const int smp = tm * 32 + acc_row(q);
+            const float d = from_bf16(s.f.H2[smp * 64 + h]) > 0.0f ? acc[q] : 0.0f;
+            const __bf16 db = to_bf16(d);
+            s.dH2[smp * 64 + h] = db;
+            s.dH2T[h * 64 + smp] = db;
         }
+    }
     __syncthreads();
-    // dW1 = dH1^T X (9 real input columns) and db1
-    for (int mb = 0; mb < 2; mb++) {
-        const f32x16 acc = mfma_nt(s.dH1T + mb * 32 * 64, 64, s.XT, 64, 64, zero16());
+    // dW2 = dH2^T H1, db2, and dH1 = dH2 W2 masked by relu'(layer 1)
+    {
+        const f32x16 acc = mfma_nt(s.dH2T + tm * 32 * 64, 64, s.H1T + tn * 32 * 64, 64, 64, zero16());
+#pragma unroll
+        for (int q = 0; q < 16; q++) g[kOffW2 + (tm * 32 + acc_row(q)) * 64 + tn * 32 + acc_col()] = acc[q];
+    }
+    if (wv == 0) {
+        float sb = 0.0f;
+        for (int i = 0; i < 64; i++) sb += from_bf16(s.dH2T[lane * 64 + i]);
+        g[kOffB2 + lane] = sb;
+    }
+    {
+        const f32x16 acc = mfma_nt(s.dH2 + tm * 32 * 64, 64, m.W2Tb + tn * 32 * 64, 64, 64, zero16());
+        const int h = tn * 32 + acc_col();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int smp = tm * 32 + acc_row(q);
+            const float d = from_bf16(s.f.H1[smp * 64 + h]) > 0.0f ? acc[q] : 0.0f;
+            s.dH1T[h * 64 + smp] = to_bf16(d);
+        }
+    }
+    __syncthreads();
+    // dW1 = dH1^T X (9 real input columns, waves 0-1) and db1 (wave 2)
+    if (wv < 2) {
+        const f32x16 acc = mfma_nt(s.dH1T + wv * 32 * 64, 64, s.XT, 64, 64, zero16());
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             const int i = acc_col();
-            if (i < 9) g[kOffW1 + (mb * 32 + acc_row(q)) * 9 + i] = acc[q];
+            if (i < 9) g[kOffW1 + (wv * 32 + acc_row(q)) * 9 + i] = acc[q];
         }
+    } else if (wv == 2) {
+        float sb = 0.0f;
+        for (int i = 0; i < 64; i++) sb += from_bf16(s.dH1T[lane * 64 + i]);
+        g[kOffB1 + lane] = sb;
+    } else if (lane == 0) {
+        lossOut[blockIdx.x] = lossW[0] + lossW[1] + lossW[2] + lossW[3];
     }
-    {
-        float sum = 0.0f;
-        for (int i = 0; i < 64; i++) sum += from_bf16(s.dH1T[lane * 64 + i]);
-        g[kOffB1 + lane] = sum;
-    }
-    // the chunk's loss (lower lanes hold the target terms)
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-    if (lane == 0) lossOut[blockIdx.x] = loss;
 }
 
-// sum the chunk gradients: grad[p] = sum_c partial[c][p]
+// sum the chunk gradients: grad[p] = sum_c partial[c][p].  A block takes
+// 64 parameters; its four waves sum interleaved quarters of the chunks
+// (chunk c to wave c % 4) and the quarters are added in wave order.
 __global__ __launch_bounds__(256) void k_grad_reduce(const float* __restrict__ partial, uint32_t chunks,
                                                      float* __restrict__ grad)
 {
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= (uint32_t)kMlpParams) return;
+    __shared__ float part[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t p = blockIdx.x * 64u + lane;
     float sum = 0.0f;
-    for (uint32_t c = 0; c < chunks; c++) sum += partial[(size_t)c * kMlpParams + p];
-    grad[p] = sum;
+    if (p < (uint32_t)kMlpParams)
+        for (uint32_t c = wv; c < chunks; c += 4u) sum += partial[(size_t)c * kMlpParams + p];
+    part[wv][lane] = sum;
+    __syncthreads();
+    if (wv == 0 && p < (uint32_t)kMlpParams) grad[p] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
 // Adam (Kingma & Ba 2015, Algorithm 1) on the FP32 master copy
@@ -670,8 +718,8 @@ hipError_t launch_mlp_grad(const MlpView& m, const float* x, const uint32_t* t, 
 {
     if (B == 0) return hipSuccess;
     const uint32_t chunks = (B + 63) / 64;
-    hipLaunchKernelGGL(k_mlp_grad, dim3(chunks), dim3(64), 0, s, m, x, t, w, B, 1.0f / (float)B, partial, loss);
-    hipLaunchKernelGGL(k_grad_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, partial, chunks, grad);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(chunks), dim3(256), 0, s, m, x, t, w, B, 1.0f / (float)B, partial, loss);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((kMlpParams + 63) / 64), dim3(256), 0, s, partial, chunks, grad);
     return hipGetLastError();
 }
 
