@@ -122,7 +122,8 @@ def quality(log_fn) -> dict:
         "workload": "c5_dynamic_1024: 16,384 emitters moving up to 3 voxels/frame, 5% of bricks edited per frame, "
                     "MIS spatial merge, 1920x1080, 120 frames along the path; learned proposal fine-tuned every 4 "
                     "frames from the plain context, plain fallback when its MSE x frame time is worse "
-                    f"(checked every 16 frames against {d['gt_spp']}-spp references)",
+                    f"(checked every 16 frames on the MSE of the last {d['window']} frames, each against its own "
+                    f"{d['gt_spp']}-spp reference)",
         "mean_mse_plain": d["mean_mse_plain"], "mean_mse_learned_context": d["mean_mse_learned_context"],
         "checks": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in c.items()} for c in d["checks"]]}
     return out

@@ -170,14 +170,16 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
 
 
 def dynamic(name: str = "c5_dynamic_1024", width=None, height=None, frames: int = 120, finetune_every: int = 4,
-            check_every: int = 16, gt_spp: int = 512, nsamples: int = 1 << 16, lr: float = 1e-3,
-            train_steps: int = 16, nclusters: int = 256, mix: float = 0.25, pretrain_frames: int = 32) -> dict:
+            check_every: int = 16, gt_spp: int = 256, nsamples: int = 1 << 16, lr: float = 1e-3,
+            train_steps: int = 16, nclusters: int = 256, mix: float = 0.25, pretrain_frames: int = 32,
+            window: int = 4) -> dict:
     """The dynamic configuration's learned-vs-plain run.  Two contexts render
     the same moving scene (identical device motion, same camera path): the
     plain power source, and the learned proposal fine-tuned every
     `finetune_every` frames from the plain context's reservoirs.  Every
-    `check_every` frames both frames are compared with a converged reference
-    of the scene as it stands (gt_spp samples); when the learned method's
+    `check_every` frames the MSE of each method over the last `window`
+    frames (each frame against a converged reference of the scene as it
+    then stood, gt_spp samples) decides: when the learned method's
     MSE x frame time is worse than the plain method's, the learned context
     falls back to the plain source (mixture weight 1, which samples exactly
     as the power source) until the next check, else it uses the proposal.
@@ -212,6 +214,7 @@ def dynamic(name: str = "c5_dynamic_1024", width=None, height=None, frames: int 
     wT.close()
     _log(f"evaluate: pretrained on {pretrain_frames} frames in {time.perf_counter() - t0:.1f} s")
     checks = []
+    win = []
     using_proposal = True
     msP = msL = 0.0
     nP = nL = 0
@@ -233,12 +236,17 @@ def dynamic(name: str = "c5_dynamic_1024", width=None, height=None, frames: int 
             t1 = time.perf_counter()
             rL.mlp_train(nsamples, lr, train_steps, source=rP)
             msL += 1000.0 * (time.perf_counter() - t1)
-        if f % check_every == check_every - 1:
+        if f % check_every >= check_every - window:
+            # this frame's error, against the scene as it stands now
             wP.look_at(pos, tgt)
             p = wP.prepare_frame()
             gt, noise = converged_reference(rP, p, gt_spp)
-            eP = mse(gt, rP.frame(), W, H)
-            eL = mse(gt, rL.frame(), W, H)
+            win.append((mse(gt, rP.frame(), W, H), mse(gt, rL.frame(), W, H), noise))
+        if f % check_every == check_every - 1:
+            eP = float(np.mean([e[0] for e in win]))
+            eL = float(np.mean([e[1] for e in win]))
+            noise = float(np.mean([e[2] for e in win]))
+            win = []
             tP, tL = msP / max(nP, 1), msL / max(nL, 1)
             used = "learned" if using_proposal else "plain"
             using_proposal = eL * tL <= eP * tP
@@ -253,7 +261,7 @@ def dynamic(name: str = "c5_dynamic_1024", width=None, height=None, frames: int 
     wP.close()
     wL.close()
     return {"config": name, "resolution": [W, H], "frames": frames, "finetune_every": finetune_every,
-            "check_every": check_every, "gt_spp": gt_spp, "mix": mix, "checks": checks,
+            "check_every": check_every, "window": window, "gt_spp": gt_spp, "mix": mix, "checks": checks,
             "mean_mse_plain": float(np.mean([c["mse_plain"] for c in checks])) if checks else None,
             "mean_mse_learned_context": float(np.mean([c["mse_learned_context"] for c in checks])) if checks else None}
 
@@ -276,7 +284,7 @@ def main(argv=None) -> int:
     for name in args.configs.split(","):
         t0 = time.perf_counter()
         if CONFIGS[name].light_speed > 0:
-            res = dynamic(name, args.width, args.height, gt_spp=min(args.gt_spp, 512), mix=args.mix)
+            res = dynamic(name, args.width, args.height, gt_spp=min(args.gt_spp, 256), mix=args.mix)
         elif CONFIGS[name].clusters:
             res = equal_time(name, args.width, args.height, args.train_frames, args.eval_frames, args.gt_spp,
                              train_steps=args.train_steps, mix=args.mix, flags=flags)
