@@ -38,6 +38,13 @@ int main(int argc, char** argv)
     vrw_optimize(w);
     vrw_scene_lights_above(w, 64, 128, 1, 1, 2, 2, 20, 1, 10);
     vrw_set_focal(w, 1.7320508f);  // 60 degree field of view
+    // the scene goes to the device once before the loop (the reference
+    // renders its initial device state on the first frame instead; here
+    // vr_render_frame refuses to render before any commit)
+    if (vrw_commit(w) != VR_OK) {
+        std::fprintf(stderr, "commit: %s\n", vrw_last_error(w));
+        return 1;
+    }
     vr_render_params* p = vrw_params(w);
     p->numberOfCandidates = 32, p->spatialTaps = 5, p->spatialRadius = 30, p->numberOfMaxTemporalImportance = 20;
     for (uint32_t f = 0; f < frames; f++) {

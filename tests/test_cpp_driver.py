@@ -48,6 +48,7 @@ def test_driver_frame_matches_the_python_binding(gpu, tmp_path):
     w.optimize()
     w.scene_lights_above(64, 128, 1, 1, 2, 2, 20, 1, 10)
     w.set_focal(1.7320508)
+    w.commit()
     p = w.params
     p.numberOfCandidates, p.spatialTaps, p.spatialRadius, p.numberOfMaxTemporalImportance = 32, 5, 30, 20
     for f in range(F):
