@@ -5,7 +5,7 @@ plus 2,000 boxes), 4,096 emissive surface voxels with log-uniform intensity
 in [0.1, 10], per-face solid-angle light sampling, 1920x1080, 1 spp, M=32,
 temporal clamp 20x, 5 spatial taps within 30 px, along the seeded 60-frame
 spline camera path (frames wrap).  One step = one frame: primary brick-grid
-DDA -> 32 RIS candidates + shadow ray + temporal reuse -> spatial reuse ->
+DDA -> 32 RIS candidates + shadow ray + temporal reuse -> spatial reuse +
 shading with a second shadow ray.  The world, lights and all per-pixel state
 are resident in HBM before the timed region starts.
 
@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 #   primary: write hit word, dist, seed                               12
 #   initial: read hit 12, prev hit 8, prev reservoir 36;
 #            write reservoir 36, seed 4                                96
-#   spatial: read own reservoir 36 + hit 12; write reservoir 36, seed 4 88
-#   shade:   read hit word + dist 8, reservoir 36; write frame 16       60
-STAGE_BYTES = {"primary": 12, "initial": 96, "spatial": 88, "shade": 60}
+#   resolve (spatial reuse + shading, one kernel): read own reservoir 36
+#            + hit 12; write reservoir 36, seed 4, frame 16             104
+STAGE_BYTES = {"primary": 12, "initial": 96, "resolve": 104}
 RAYS_PER_PIXEL = 3  # primary + candidate shadow ray + shading shadow ray
 
 
@@ -320,7 +320,7 @@ def main() -> int:
 
     npx = sc.width * sc.height
     fps = args.steps / elapsed
-    stage_ms = {name: float(stage_sum[i] / max(nprof, 1)) for i, name in enumerate(("primary", "initial", "spatial", "shade"))}
+    stage_ms = {name: float(stage_sum[i] / max(nprof, 1)) for i, name in enumerate(_lib.STAGE_NAMES)}
     dom = max(stage_ms, key=stage_ms.get)
     achieved = STAGE_BYTES[dom] * npx / (stage_ms[dom] * 1e-3) / 1e9
     traffic = measured_traffic(sc.name).get(f"k_{dom}")

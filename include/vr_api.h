@@ -64,9 +64,9 @@ extern "C" {
 /* stages, in pipeline order (used by vr_stage_times) */
 #define VR_STAGE_PRIMARY 0  /* renderAlbedo             cl/ris.cl:295-389 */
 #define VR_STAGE_INITIAL 1  /* perPixelInitialSampling  cl/ris.cl:20-124  */
-#define VR_STAGE_SPATIAL 2  /* perPixelSpatialResampling cl/ris.cl:126-151 */
-#define VR_STAGE_SHADE   3  /* renderRIS                cl/ris.cl:391-413 */
-#define VR_NUM_STAGES    4
+#define VR_STAGE_RESOLVE 2  /* perPixelSpatialResampling cl/ris.cl:126-151, then
+                               renderRIS cl/ris.cl:391-413, in one kernel */
+#define VR_NUM_STAGES    3
 
 /* order in which workgroups take the 16x16 pixel tiles, per stage
  * (vr_config.flags, or vr_set_tile_order); results do not depend on it */
@@ -115,8 +115,9 @@ typedef struct vr_config {
     uint32_t quirks;          /* VR_QUIRK_* */
     uint32_t source_pdf;      /* VR_SOURCE_* */
     uint32_t brick_capacity;  /* device brick pool size in bricks (reference: VR_BRICKCOUNT) */
-    uint32_t flags;           /* bits 2s..2s+1: tile order VR_TILES_* of stage s;
-                                 VR_FLAG_MIS_SPATIAL; other bits zero */
+    uint32_t flags;           /* bits 2s..2s+1: tile order VR_TILES_* of stage s
+                                 (s < VR_NUM_STAGES); VR_FLAG_MIS_SPATIAL; other
+                                 bits zero */
     float emit_lut[256];      /* EmitStrength(v) = emit_lut[(v >> 16) & 255]; the
                                  reference's is emit_lut[m] = m (common.h:208-211) */
 } vr_config;

@@ -25,8 +25,8 @@ TRACE_ALL, TRACE_OPAQUE = 0, 1
 TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS = 0, 1, 2
 FLAG_MIS_SPATIAL, MIS_MAX_TAPS = 1 << 8, 32
 MAX_VIEWS = 64
-NUM_STAGES = 4
-STAGE_NAMES = ("primary", "initial", "spatial", "shade")
+NUM_STAGES = 3
+STAGE_NAMES = ("primary", "initial", "resolve")  # resolve = spatial reuse + shading
 GRIDSIZE = 128 ** 3
 BRICKSIZE = 512
 BRICKCOUNT = 1 << 20

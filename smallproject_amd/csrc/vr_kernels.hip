@@ -1,12 +1,15 @@
 // vr_kernels.hip - gfx950 kernels of the Voxel ReSTIR frame.
 //
-// One frame = four launches on the context stream, each one thread per pixel
-// with 256-thread workgroups covering a 16x16 pixel tile (four 8x8 wave64
-// tiles, so the rays of a wave are spatially coherent):
+// One frame = three launches on the context stream, each one thread per
+// pixel with 256-thread workgroups covering a 16x16 pixel tile (four 8x8
+// wave64 tiles, so the rays of a wave are spatially coherent):
 //   k_primary   renderAlbedo                cl/ris.cl:295-389
 //   k_initial   perPixelInitialSampling     cl/ris.cl:20-124
-//   k_spatial   perPixelSpatialResampling   cl/ris.cl:126-151 + spatialSampling.cl
-//   k_shade     renderRIS / render_di_ris   cl/ris.cl:155-293, 391-413
+//   k_resolve   perPixelSpatialResampling   cl/ris.cl:126-151 + spatialSampling.cl
+//               then renderRIS / render_di_ris cl/ris.cl:155-293, 391-413
+// Shading reads only its own pixel's final reservoir, which the spatial
+// pass of the same thread has just written (no other pixel's spatial pass
+// reads it), so the two run as one kernel.
 // Per-pixel state lives in structure-of-arrays planes (12 B of hit data and
 // 36 B per reservoir) so every pass streams coalesced 4/16-byte words.
 #include "vr_kernels.h"
@@ -162,18 +165,24 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
     for (int a = 0; a < 3; a++) total += fidx[a] == a ? sol[a] : 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; a++) total += fidx[a] == a + 3 ? sol[a] : 0.0f;
+    // total is finite and >= 0.  The walk's pdf starts at 0/total: 0, or
+    // NaN for total = 0; either way the weight below is 0, so 0 is kept.
     int index = 5;
-    float pdf = 0.0f / total;  // face 5 when no front face reaches r2
+    float pdf = 0.0f;
     if (total == 0.0f) {
         index = 0;  // 0/0: the walk stops at face 0 with pdf NaN
     } else {
+        // the two faces of an axis share its solid angle: one division each
+        float q[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) q[a] = sol[a] / total;
         float cdf = 0.0f;
         bool done = false;
 #pragma unroll
         for (int k = 0; k < 6; k++) {
             const int a = k % 3;
             if (!done && fidx[a] == k) {
-                const float p = sol[a] / total;
+                const float p = q[a];
                 cdf = cdf + p;
                 if (cdf >= r2 || k == 5) index = k, pdf = p, done = cdf >= r2;
             }
@@ -626,8 +635,8 @@ __device__ __forceinline__ void many_views(const ViewChunk& c, F f)
 // Occupancy targets (waves per SIMD) of the ray-tracing stages: their DDA
 // loops are bound by dependent-load latency, and more waves in flight hide
 // more of it than the few registers the compiler then spills cost
-// (measured on c2 at 1080p: candidates 1.83 -> 1.68 ms at 7 waves,
-// shading 0.58 -> 0.51 ms at 8, the MIS spatial merge 3.17 -> 2.59 ms at 8).
+// (measured on c2 at 1080p: candidates 1.83 -> 1.68 ms at 7 waves; spatial
+// reuse + shading 0.557 ms at 8, 0.541 ms at 6).
 #define VR_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
 // One-view stage kernels, instantiated for occupancy targets of 5-8 waves
@@ -644,14 +653,12 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
     one_view(a, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
 }
 template <bool kMis, int W>
-__global__ __launch_bounds__(256) VR_WAVES(W) void k_spatial(FrameArgs a)
+__global__ __launch_bounds__(256) VR_WAVES(W) void k_resolve(FrameArgs a)
 {
-    one_view<kMis>(a, [](const FrameArgs& v, int x, int y) { spatial_px<kMis>(v, x, y); });
-}
-template <int W>
-__global__ __launch_bounds__(256) VR_WAVES(W) void k_shade(FrameArgs a)
-{
-    one_view(a, [](const FrameArgs& v, int x, int y) { shade_px(v, x, y); });
+    one_view(a, [](const FrameArgs& v, int x, int y) {
+        spatial_px<kMis>(v, x, y);
+        shade_px(v, x, y);
+    });
 }
 
 template <bool kPersistent>
@@ -667,17 +674,13 @@ __global__ __launch_bounds__(256) VR_WAVES(7) void k_initial_views(ViewChunk c)
 // kAnyMis: some view merges with MIS weights (the kernel then carries the
 // MIS variant's registers)
 template <bool kPersistent, bool kAnyMis>
-__global__ __launch_bounds__(256) VR_WAVES(8) void k_spatial_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(6) void k_resolve_views(ViewChunk c)
 {
     many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) {
         if (kAnyMis && v.misSpatial) spatial_px<true>(v, x, y);
         else spatial_px<false>(v, x, y);
+        shade_px(v, x, y);
     });
-}
-template <bool kPersistent>
-__global__ __launch_bounds__(256) VR_WAVES(8) void k_shade_views(ViewChunk c)
-{
-    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { shade_px(v, x, y); });
 }
 
 // ---------------------------------------------------------------------------
@@ -805,9 +808,8 @@ static const void* stage_kernel_w(int stage, bool mis)
     switch (stage) {
     case 0: return reinterpret_cast<const void*>(&k_primary<W>);
     case 1: return reinterpret_cast<const void*>(&k_initial<W>);
-    case 2: return mis ? reinterpret_cast<const void*>(&k_spatial<true, W>)
-                       : reinterpret_cast<const void*>(&k_spatial<false, W>);
-    case 3: return reinterpret_cast<const void*>(&k_shade<W>);
+    case 2: return mis ? reinterpret_cast<const void*>(&k_resolve<true, W>)
+                       : reinterpret_cast<const void*>(&k_resolve<false, W>);
     default: return nullptr;
     }
 }
@@ -815,7 +817,7 @@ static const void* stage_kernel_w(int stage, bool mis)
 // waves 0 = the stage's default target
 static const void* stage_kernel(int stage, bool mis, uint32_t waves)
 {
-    if (waves == 0) waves = kDefaultWaves[stage < 0 || stage > 3 ? 0 : stage];
+    if (waves == 0) waves = kDefaultWaves[stage < 0 || stage > 2 ? 0 : stage];
     switch (waves) {
     case 5: return stage_kernel_w<5>(stage, mis);
     case 6: return stage_kernel_w<6>(stage, mis);
@@ -840,9 +842,8 @@ static const void* stage_views_kernel_t(int stage, bool anyMis)
     case 0: return reinterpret_cast<const void*>(&k_primary_views<kP>);
     case 1: return reinterpret_cast<const void*>(&k_initial_views<kP>);
     case 2:
-        return anyMis ? reinterpret_cast<const void*>(&k_spatial_views<kP, true>)
-                      : reinterpret_cast<const void*>(&k_spatial_views<kP, false>);
-    case 3: return reinterpret_cast<const void*>(&k_shade_views<kP>);
+        return anyMis ? reinterpret_cast<const void*>(&k_resolve_views<kP, true>)
+                      : reinterpret_cast<const void*>(&k_resolve_views<kP, false>);
     default: return nullptr;
     }
 }

@@ -422,7 +422,7 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     if (cfg->brick_capacity == 0 || cfg->brick_capacity > (1u << 22)) return VR_ERR_ARG;
     if (cfg->source_pdf > VR_SOURCE_LEARNED || cfg->light_mode > VR_LIGHTS_POINT) return VR_ERR_ARG;
     if (cfg->emit_lut[0] != 0.0f) return VR_ERR_ARG;
-    if (cfg->flags & ~(0xffu | VR_FLAG_MIS_SPATIAL)) return VR_ERR_ARG;
+    if (cfg->flags & ~(((1u << (2 * VR_NUM_STAGES)) - 1u) | VR_FLAG_MIS_SPATIAL)) return VR_ERR_ARG;
     for (int s = 0; s < VR_NUM_STAGES; s++)
         if (((cfg->flags >> (2 * s)) & 3u) > VR_TILES_XCD_COLS) return VR_ERR_ARG;
     vr_ctx* c = new (std::nothrow) vr_ctx();

@@ -78,11 +78,13 @@ public:
     void PrepareFrame();
     void FinishFrame();
 
-    // per-stage timing of the last frame, ms (template/world.h:222-227)
+    // per-stage timing of the last frame, ms (template/world.h:222-227).
+    // Spatial reuse and shading run as one kernel: GetSpatialTime() covers
+    // both and GetShadingTime() is 0.
     float GetAlbedoTime() const { return stageMs[VR_STAGE_PRIMARY]; }
     float GetCandidateTime() const { return stageMs[VR_STAGE_INITIAL]; }
-    float GetSpatialTime() const { return stageMs[VR_STAGE_SPATIAL]; }
-    float GetShadingTime() const { return stageMs[VR_STAGE_SHADE]; }
+    float GetSpatialTime() const { return stageMs[VR_STAGE_RESOLVE]; }
+    float GetShadingTime() const { return 0.0f; }
     float GetRenderTime() const;
 
     // low-level voxel access (template/world.h:293-513)
@@ -155,7 +157,7 @@ private:
     uint32_t rng = 0x12345678u;  // RandomUInt state, template/template.cpp:933-941
     bool r0Override = false;
     uint32_t r0Next = 0;
-    float stageMs[VR_NUM_STAGES] = {0, 0, 0, 0};
+    float stageMs[VR_NUM_STAGES] = {};
     int status = VR_OK;
     std::string error;
 };
