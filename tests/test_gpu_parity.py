@@ -1,10 +1,10 @@
 """HIP path vs the CPU oracle, through the C ABI, on the same seeded inputs.
 
-Integer outputs (hit voxel word, side and face, seeds, reservoir M, light
-index, Nlight face) must be bit-exact; float outputs (dist, wsum, W, pHat,
-invPdf, sample position, radiance) must satisfy |a-b| <= 1e-5|b| + 1e-7 on
-at least 99.9% of pixels (the kernels follow the oracle's operation order,
-so they are expected to be bit-exact as well; the report prints both).
+Every output must be bit-exact on every pixel: the integer ones (hit voxel
+word, side and face, seeds, reservoir M, light index, Nlight face) and the
+float ones (dist, wsum, W, pHat, invPdf, sample position, radiance), since
+the kernels and the oracle follow one IEEE binary32 operation order with
+correctly rounded division and square root and no contraction.
 """
 import numpy as np
 import pytest
