@@ -507,9 +507,14 @@ __device__ __forceinline__ void spatial_px(const FrameArgs& a, int x, int y)
 }
 
 // ---------------------------------------------------------------------------
-// k_shade: final shading (cl/ris.cl:155-293 with sky dome and editor off)
+// shading (cl/ris.cl:155-293 with sky dome and editor off)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void shade_px(const FrameArgs& a, int x, int y)
+// kDefer: a pixel whose sample needs a shadow ray that the candidate pass
+// did not trace is left alone (no frame or reservoir write) and true is
+// returned; shading it again with kDefer = false gives the one-pass result,
+// because nothing it reads has changed.
+template <bool kDefer = false>
+__device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
 {
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t hitw = a.cur.hitw[idx];
@@ -535,6 +540,7 @@ __device__ __forceinline__ void shade_px(const FrameArgs& a, int x, int y)
                     const float4 la = a.L.a[res.light], lb = a.L.b[res.light];
                     const f3 L = normalize(sub(res.pos, spo));
                     const int known = tested_visibility(a.tested[idx], res.light, res.pos);
+                    if (kDefer && known < 0) return true;
                     const bool visible =
                         known >= 0 ? known == 1
                                    : !light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
@@ -558,6 +564,7 @@ __device__ __forceinline__ void shade_px(const FrameArgs& a, int x, int y)
         }
     }
     a.frame[idx] = make_float4(color.x, color.y, color.z, dist);
+    return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -599,7 +606,9 @@ __device__ __forceinline__ void one_view(const FrameArgs& a, F f)
 // tile t of the chunk; `staged` is the view whose uber grid LDS holds
 // (~0u: none yet).  Uniform control flow: every thread of the block runs
 // the staging barriers.
-template <class F>
+// kAll: f runs on every thread of the tile (it checks the bounds itself and
+// may use barriers); otherwise only on the pixels inside the view.
+template <bool kAll = false, class F>
 __device__ __forceinline__ void view_tile(const ViewChunk& c, uint32_t t, uint32_t& staged, F f)
 {
     uint32_t v = 0;
@@ -616,27 +625,27 @@ __device__ __forceinline__ void view_tile(const ViewChunk& c, uint32_t t, uint32
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int x = (int)(lt % tilesX) * 16 + (wv & 1) * 8 + (lane & 7);
     const int y = (int)(lt / tilesX) * 16 + (wv >> 1) * 8 + (lane >> 3);
-    if (x < (int)a.W && y < (int)a.H) f(a, x, y);
+    if (kAll || (x < (int)a.W && y < (int)a.H)) f(a, x, y);
 }
 
 // kPersistent: gridDim.x < total, each block loops; otherwise one tile per block
-template <bool kPersistent, class F>
+template <bool kPersistent, bool kAll = false, class F>
 __device__ __forceinline__ void many_views(const ViewChunk& c, F f)
 {
     const uint32_t total = c.start[c.n];
     uint32_t staged = ~0u;
     if (!kPersistent) {
-        if (blockIdx.x < total) view_tile(c, blockIdx.x, staged, f);
+        if (blockIdx.x < total) view_tile<kAll>(c, blockIdx.x, staged, f);
         return;
     }
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) view_tile(c, t, staged, f);
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) view_tile<kAll>(c, t, staged, f);
 }
 
 // Occupancy targets (waves per SIMD) of the ray-tracing stages: their DDA
 // loops are bound by dependent-load latency, and more waves in flight hide
 // more of it than the few registers the compiler then spills cost
 // (measured on c2 at 1080p: candidates 1.83 -> 1.68 ms at 7 waves; spatial
-// reuse + shading 0.557 ms at 8, 0.541 ms at 6).
+// reuse + shading 0.537 ms at 6, 0.525 ms at 7).
 #define VR_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
 // One-view stage kernels, instantiated for occupancy targets of 5-8 waves
@@ -652,13 +661,51 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
 }
+// Spatial reuse, then shading of the tile's 256 pixels.  Only the pixels
+// whose final sample needs a shadow ray the candidate pass did not trace
+// (about one in seven on c2) trace one, and a wave would wait for the
+// longest of its few: the tile's ray pixels are gathered in LDS (in pixel
+// order) and shaded by the first threads, so they fill as few waves as
+// possible and the other waves finish at once.  Uniform control flow: every
+// thread reaches both barriers.
+template <bool kMis>
+__device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
+{
+    __shared__ uint32_t rayPx[256];  // threads of the tile whose pixel needs a ray
+    __shared__ uint32_t waveRays[4];
+    bool defer = false;
+    if (x < (int)a.W && y < (int)a.H) {
+        spatial_px<kMis>(a, x, y);
+        defer = shade_px<true>(a, x, y);
+    }
+    const uint64_t m = __ballot(defer);
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    if (lane == 0u) waveRays[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t first = 0u, n = 0u;
+    for (uint32_t k = 0; k < 4u; k++) {
+        first += k < wv ? waveRays[k] : 0u;
+        n += waveRays[k];
+    }
+    if (defer) rayPx[first + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = threadIdx.x;
+    __syncthreads();
+    if (threadIdx.x < n) {
+        // the queued thread's pixel: same tile, its own wave quadrant and lane
+        const uint32_t q = rayPx[threadIdx.x], qw = q >> 6, ql = q & 63u;
+        const int x0 = x - (int)((wv & 1u) * 8u + (lane & 7u)), y0 = y - (int)((wv >> 1) * 8u + (lane >> 3));
+        shade_px(a, x0 + (int)((qw & 1u) * 8u + (ql & 7u)), y0 + (int)((qw >> 1) * 8u + (ql >> 3)));
+    }
+}
+
 template <bool kMis, int W>
 __global__ __launch_bounds__(256) VR_WAVES(W) void k_resolve(FrameArgs a)
 {
-    one_view(a, [](const FrameArgs& v, int x, int y) {
-        spatial_px<kMis>(v, x, y);
-        shade_px(v, x, y);
-    });
+    stage_uber(a.w.uber);
+    int x, y;
+    tile_pixel(a.tileOrder, x, y);
+    FrameArgs b = a;
+    b.w.uber = uber_lds();
+    resolve_tile<kMis>(b, x, y);
 }
 
 template <bool kPersistent>
@@ -674,12 +721,11 @@ __global__ __launch_bounds__(256) VR_WAVES(7) void k_initial_views(ViewChunk c)
 // kAnyMis: some view merges with MIS weights (the kernel then carries the
 // MIS variant's registers)
 template <bool kPersistent, bool kAnyMis>
-__global__ __launch_bounds__(256) VR_WAVES(6) void k_resolve_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(7) void k_resolve_views(ViewChunk c)
 {
-    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) {
-        if (kAnyMis && v.misSpatial) spatial_px<true>(v, x, y);
-        else spatial_px<false>(v, x, y);
-        shade_px(v, x, y);
+    many_views<kPersistent, true>(c, [](const FrameArgs& v, int x, int y) {
+        if (kAnyMis && v.misSpatial) resolve_tile<true>(v, x, y);
+        else resolve_tile<false>(v, x, y);
     });
 }
 
