@@ -25,7 +25,11 @@ CXX_SOURCES = ["world.cpp", "world_capi.cpp", "scenes.cpp", "vxio.cpp", "nbtio.c
 
 COMMON = ["-O3", "-std=c++17", "-ffp-contract=off", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}"]
 # Packed FP32 (v_pk_*_f32) instructions are disabled for all device code.
-HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# -Wno-pass-failed: the MIS merge's stage kernel is also instantiated for 7
+# and 8 waves per SIMD, which its LDS and registers cannot reach; the
+# compiler then builds it for the occupancy it can reach (4), as intended.
+HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-pass-failed"]
 
 
 def hipcc() -> str:

@@ -80,8 +80,10 @@ __device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
 }
 
 // default occupancy targets (waves per SIMD) of the stage kernels: primary,
-// candidates, spatial reuse + shading
+// candidates, spatial reuse + shading; the MIS merge's variant of the last
+// holds its tile's ray list in LDS and more live values, and runs best at 5
 constexpr uint32_t kDefaultWaves[3] = {8, 7, 7};
+constexpr uint32_t kDefaultWavesMis = 5;
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s);
 // Up to kChunkViews views rendered by one launch per stage: their
 // FrameArgs travel in the kernel argument (8 x 448 B + the tile table,

@@ -403,107 +403,201 @@ __device__ __forceinline__ void tap_offset(uint32_t& seed, float rad, int& dx, i
     dx = f2i(cs * radius), dy = f2i(sn * radius);
 }
 
+// The spatial merge of one pixel (cl/spatialSampling.cl:2-67).  `res` starts
+// as the pixel's B reservoir; with a hit and spatial reuse on (`merged`)
+// it is merged with the accepted taps and adjusted, `seed` is the stream
+// after the taps, and the MIS merge's bookkeeping is kept: M0 (own M),
+// seed0 (the stream before the taps), `mask` (combined taps) and `sel`
+// (tap whose sample was taken last, -1 = own).
+struct Merge {
+    Res res;
+    Shading s;
+    uint32_t hitw, seed, seed0, M0, mask;
+    int sel;
+    bool merged;
+};
+
+template <bool kMis>
+__device__ __forceinline__ Merge spatial_merge(const FrameArgs& a, int x, int y)
+{
+    Merge g;
+    const size_t idx = (size_t)y * a.W + x;
+    g.res = load_res(a.B, idx);
+    g.hitw = a.cur.hitw[idx];
+    g.mask = 0u, g.sel = -1;
+    g.merged = (g.hitw & 0xffffffu) != 0u && a.spatial;
+    if (!g.merged) return g;
+    Res& res = g.res;
+    const float dist = a.cur.dist[idx];
+    uint32_t seed = a.cur.seed[idx];
+    g.seed0 = seed, g.M0 = res.M;
+    g.s = shading_at(a, x, y, g.hitw, dist);
+    const Shading& s = g.s;
+    const float rad = (float)(a.spatialRadius - 1u);
+    for (uint32_t i = 0; i < a.spatialTaps; i++) {
+        int dx, dy;
+        tap_offset(seed, rad, dx, dy);
+        const int nx = x + dx, ny = y + dy;
+        if ((nx == x && ny == y) || nx < 0 || ny < 0 || nx >= (int)a.W || ny >= (int)a.H) continue;
+        const size_t ni = (size_t)ny * a.W + nx;
+        // the neighbour's 32-byte record: one sector instead of five planes
+        const uint4 n0 = a.nbr[2 * ni];
+        if ((n0.x & 0xffffffu) == 0u) continue;
+        const float nd = __uint_as_float(n0.y);
+        if (!same_face((n0.x >> 3) & (7u << 24), g.hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist))
+            continue;
+        const uint4 n1 = a.nbr[2 * ni + 1];
+        Res nr;
+        nr.M = n0.x & 0xffffffu, nr.face = (n0.x >> 24) & 7u;
+        nr.W = __uint_as_float(n0.z), nr.invPdf = __uint_as_float(n0.w);
+        nr.pos = f3{__uint_as_float(n1.x), __uint_as_float(n1.y), __uint_as_float(n1.z)};
+        nr.light = n1.w;
+        nr.wsum = 0.0f, nr.pHat = 0.0f;  // both are recomputed by the reweigh below
+        if (nr.light >= a.L.n) continue;
+        const float4 la = a.L.a[nr.light], lb = a.L.b[nr.light];
+        const float ph = phat(a.point, nr.pos, face_normal(nr.face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) *
+                         nr.invPdf;
+        res_reweigh(nr, ph, nr.M);
+        const float r0 = rnd(seed);
+        if (kMis) {
+            // UpdateReservoir's test: does the neighbour's sample take over?
+            if (r0 < nr.wsum / fmaxf(1e-5f, res.wsum + nr.wsum)) g.sel = (int)i;
+            g.mask |= 1u << i;
+        }
+        res_combine(res, nr, r0);
+    }
+    res_adjust(res, res.pHat);
+    g.seed = seed;
+    return g;
+}
+
+// the reference's merge: merge, then store the final reservoir (and the
+// stream) for shading
+__device__ __forceinline__ void spatial_px(const FrameArgs& a, int x, int y)
+{
+    const Merge g = spatial_merge<false>(a, x, y);
+    const size_t idx = (size_t)y * a.W + x;
+    if (g.merged) a.cur.seed[idx] = g.seed;
+    store_res(a.A, idx, g.res);
+}
+
 // Unbiased contribution weight of the merged sample under generalized
 // balance-heuristic MIS (weights evaluated for the selected sample only):
 //     W = wsum * T_sel(y) / (pHat * sum_s M_s T_s(y))
 // over the own reservoir (s = own, M0, T = phat without a ray: shading
 // multiplies by the own visibility) and the combined neighbours in tap
 // order (T = phat at the neighbour's surface point times its visibility of
-// y, one shadow ray each).  The neighbours are found again by replaying the
-// tap draws from seed0: `mask` marks the combined taps, each of which drew
-// one more number for its merge.  The common invPdf of y cancels.
-__device__ __forceinline__ float mis_weight(const FrameArgs& a, int x, int y, const Shading& s, const Res& res,
-                                         uint32_t M0, uint32_t seed0, uint32_t mask, int sel)
+// y, one shadow ray each unless the candidate pass traced that ray).  The
+// neighbours are found again by replaying the tap draws from seed0: `mask`
+// marks the combined taps, each of which drew one more number for its
+// merge.  The common invPdf of y cancels.
+//
+// The visibility rays of a 16x16 tile are traced together: the taps are
+// taken kMisChunk at a time; each pixel lists the rays its combined taps of
+// the chunk need in LDS, all 256 threads trace the list, and each pixel then
+// adds its chunk's terms in tap order.  The rays of a pixel no longer run
+// one after another in its lane while other lanes idle.  Uniform control
+// flow: every thread runs every chunk's barriers.
+constexpr uint32_t kMisChunk = 5;
+
+struct MisLds {
+    float4 sample[256];               // the thread's merged sample: pos.xyz, light bits
+    float term[kMisChunk][256];       // M_s * T_s(y) of tap k of the chunk, before its ray
+    uint32_t occluded[256];           // bit k: tap k of the chunk found the sample occluded
+    uint32_t task[kMisChunk * 256];   // owner thread | tap of the chunk << 8
+    uint32_t taskPx[kMisChunk * 256]; // the tap's neighbour pixel
+    uint32_t count[2];                // list length, per chunk parity
+};
+
+__device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int y, const Merge& g, bool in)
 {
-    if (!(res.wsum > 0.0f) || res.light >= a.L.n) return 0.0f;
-    const float4 la = a.L.a[res.light], lb = a.L.b[res.light];
-    const f3 Le = f3{lb.x, lb.y, lb.z};
-    const f3 Nl = face_normal(res.face);
-    const float p0 = phat(a.point, res.pos, Nl, s.sp, s.N, s.brdf, Le, la.w);
-    float den = (float)M0 * p0;
-    float psel = sel < 0 ? p0 : 0.0f;
-    const float rad = (float)(a.spatialRadius - 1u);
-    uint32_t rs = seed0;
-    for (uint32_t i = 0; i < a.spatialTaps && mask >> i; i++) {
-        int dx, dy;
-        tap_offset(rs, rad, dx, dy);
-        if (!((mask >> i) & 1u)) continue;
-        (void)rnd(rs);  // the merge's draw
-        const int nx = x + dx, ny = y + dy;
-        const size_t ni = (size_t)ny * a.W + nx;
-        const uint32_t nh = a.cur.hitw[ni];
-        const uint4 n0 = a.nbr[2 * ni];
-        const Shading t = shading_at(a, nx, ny, nh, __uint_as_float(n0.y));
-        float pj = phat(a.point, res.pos, Nl, t.sp, t.N, t.brdf, Le, la.w);
-        if (pj > 0.0f) {
-            const int known = tested_visibility(a.tested[ni], res.light, res.pos);
-            if (known == 0) {
-                pj = 0.0f;
-            } else if (known < 0) {
-                const f3 spo = f3{t.sp.x + 0.1f * t.N.x, t.sp.y + 0.1f * t.N.y, t.sp.z + 0.1f * t.N.z};
-                const f3 L = normalize(sub(res.pos, spo));
-                if (light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps)) pj = 0.0f;
-            }
-        }
-        den += (float)(n0.x & 0xffffffu) * pj;
-        if ((int)i == sel) psel = pj;
+    __shared__ MisLds L;
+    const uint32_t tid = threadIdx.x;
+    const Res& res = g.res;
+    const bool need = in && g.merged && res.wsum > 0.0f && res.light < a.L.n;
+    float4 la = make_float4(0.0f, 0.0f, 0.0f, 0.0f), lb = la;
+    f3 Nl = f3{0.0f, 0.0f, 0.0f};
+    float den = 0.0f, psel = 0.0f;
+    if (need) {
+        la = a.L.a[res.light], lb = a.L.b[res.light];
+        Nl = face_normal(res.face);
+        const float p0 = phat(a.point, res.pos, Nl, g.s.sp, g.s.N, g.s.brdf, f3{lb.x, lb.y, lb.z}, la.w);
+        den = (float)g.M0 * p0;
+        psel = g.sel < 0 ? p0 : 0.0f;
     }
+    L.sample[tid] = make_float4(res.pos.x, res.pos.y, res.pos.z, __uint_as_float(need ? res.light : 0u));
+    L.occluded[tid] = 0u;
+    if (tid < 2u) L.count[tid] = 0u;
+    __syncthreads();
+    const float rad = (float)(a.spatialRadius - 1u);
+    uint32_t rs = g.seed0;
+    uint32_t c = 0;
+    for (uint32_t i0 = 0; i0 < a.spatialTaps; i0 += kMisChunk, c++) {
+        uint32_t* count = &L.count[c & 1u];
+        if (tid == 0u) L.count[(c + 1u) & 1u] = 0u;  // the next chunk's list (the last reader was chunk c - 1)
+        uint32_t combined = 0u, traced = 0u;
+        float pselChunk = 0.0f;  // T of tap `sel` if it is in this chunk, before its ray
+#pragma unroll
+        for (uint32_t k = 0; k < kMisChunk; k++) {
+            const uint32_t i = i0 + k;
+            if (!need || i >= a.spatialTaps || !(g.mask >> i)) continue;
+            int dx, dy;
+            tap_offset(rs, rad, dx, dy);
+            if (!((g.mask >> i) & 1u)) continue;
+            (void)rnd(rs);  // the merge's draw
+            combined |= 1u << k;
+            const int nx = x + dx, ny = y + dy;
+            const size_t ni = (size_t)ny * a.W + nx;
+            const uint32_t nh = a.cur.hitw[ni];
+            const uint4 n0 = a.nbr[2 * ni];
+            const Shading t = shading_at(a, nx, ny, nh, __uint_as_float(n0.y));
+            float p = phat(a.point, res.pos, Nl, t.sp, t.N, t.brdf, f3{lb.x, lb.y, lb.z}, la.w);
+            if (p > 0.0f) {
+                const int known = tested_visibility(a.tested[ni], res.light, res.pos);
+                if (known == 0) {
+                    p = 0.0f;
+                } else if (known < 0) {
+                    const uint32_t slot = atomicAdd(count, 1u);
+                    L.task[slot] = tid | (k << 8);
+                    L.taskPx[slot] = (uint32_t)ni;
+                    traced |= 1u << k;
+                }
+            }
+            // the term as the one-lane loop adds it, (float)M * T: an occluded
+            // tap adds (float)M * 0 = +0 instead
+            L.term[k][tid] = (float)(n0.x & 0xffffffu) * p;
+            if ((int)i == g.sel) pselChunk = p;
+        }
+        __syncthreads();
+        const uint32_t n = *count;
+        for (uint32_t t = tid; t < n; t += 256u) {
+            const uint32_t tk = L.task[t], ni = L.taskPx[t], owner = tk & 255u;
+            const float4 smp = L.sample[owner];
+            const uint32_t light = __float_as_uint(smp.w);
+            const int nx = (int)(ni % a.W), ny = (int)(ni / a.W);
+            const float nd = __uint_as_float(a.nbr[2 * (size_t)ni].y);
+            const Shading sh = shading_at(a, nx, ny, a.cur.hitw[ni], nd);
+            const f3 spo = f3{sh.sp.x + 0.1f * sh.N.x, sh.sp.y + 0.1f * sh.N.y, sh.sp.z + 0.1f * sh.N.z};
+            const f3 Ld = normalize(sub(f3{smp.x, smp.y, smp.z}, spo));
+            const float4 tla = a.L.a[light], tlb = a.L.b[light];
+            if (light_occluded(a.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, a.shadowSteps))
+                atomicOr(&L.occluded[owner], 1u << (tk >> 8));
+        }
+        __syncthreads();
+        const uint32_t occ = L.occluded[tid];
+        L.occluded[tid] = 0u;  // tracers of the next chunk write only after its first barrier
+#pragma unroll
+        for (uint32_t k = 0; k < kMisChunk; k++) {
+            if (!((combined >> k) & 1u)) continue;
+            const bool blocked = ((traced >> k) & 1u) && ((occ >> k) & 1u);
+            den += blocked ? 0.0f : L.term[k][tid];
+            if ((int)(i0 + k) == g.sel) psel = blocked ? 0.0f : pselChunk;
+        }
+    }
+    if (!need) return 0.0f;
     const float d = res.pHat * den;
     const float W = (res.wsum * psel) / d;
     return (d > 0.0f && isfinite(W)) ? W : 0.0f;
-}
-
-template <bool kMis>
-__device__ __forceinline__ void spatial_px(const FrameArgs& a, int x, int y)
-{
-    const size_t idx = (size_t)y * a.W + x;
-    Res res = load_res(a.B, idx);
-    const uint32_t hitw = a.cur.hitw[idx];
-    if ((hitw & 0xffffffu) != 0u && a.spatial) {
-        const float dist = a.cur.dist[idx];
-        uint32_t seed = a.cur.seed[idx];
-        const uint32_t seed0 = seed, M0 = res.M;
-        uint32_t mask = 0u;  // combined taps (MIS merge)
-        int sel = -1;        // tap whose sample was taken last, -1 = own
-        const Shading s = shading_at(a, x, y, hitw, dist);
-        const float rad = (float)(a.spatialRadius - 1u);
-        for (uint32_t i = 0; i < a.spatialTaps; i++) {
-            int dx, dy;
-            tap_offset(seed, rad, dx, dy);
-            const int nx = x + dx, ny = y + dy;
-            if ((nx == x && ny == y) || nx < 0 || ny < 0 || nx >= (int)a.W || ny >= (int)a.H) continue;
-            const size_t ni = (size_t)ny * a.W + nx;
-            // the neighbour's 32-byte record: one sector instead of five planes
-            const uint4 n0 = a.nbr[2 * ni];
-            if ((n0.x & 0xffffffu) == 0u) continue;
-            const float nd = __uint_as_float(n0.y);
-            if (!same_face((n0.x >> 3) & (7u << 24), hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist))
-                continue;
-            const uint4 n1 = a.nbr[2 * ni + 1];
-            Res nr;
-            nr.M = n0.x & 0xffffffu, nr.face = (n0.x >> 24) & 7u;
-            nr.W = __uint_as_float(n0.z), nr.invPdf = __uint_as_float(n0.w);
-            nr.pos = f3{__uint_as_float(n1.x), __uint_as_float(n1.y), __uint_as_float(n1.z)};
-            nr.light = n1.w;
-            nr.wsum = 0.0f, nr.pHat = 0.0f;  // both are recomputed by the reweigh below
-            if (nr.light >= a.L.n) continue;
-            const float4 la = a.L.a[nr.light], lb = a.L.b[nr.light];
-            const float ph = phat(a.point, nr.pos, face_normal(nr.face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) *
-                             nr.invPdf;
-            res_reweigh(nr, ph, nr.M);
-            const float r0 = rnd(seed);
-            if (kMis) {
-                // UpdateReservoir's test: does the neighbour's sample take over?
-                if (r0 < nr.wsum / fmaxf(1e-5f, res.wsum + nr.wsum)) sel = (int)i;
-                mask |= 1u << i;
-            }
-            res_combine(res, nr, r0);
-        }
-        res_adjust(res, res.pHat);
-        if (kMis) res.W = mis_weight(a, x, y, s, res, M0, seed0, mask, sel);
-        a.cur.seed[idx] = seed;
-    }
-    store_res(a.A, idx, res);
 }
 
 // ---------------------------------------------------------------------------
@@ -645,7 +739,8 @@ __device__ __forceinline__ void many_views(const ViewChunk& c, F f)
 // loops are bound by dependent-load latency, and more waves in flight hide
 // more of it than the few registers the compiler then spills cost
 // (measured on c2 at 1080p: candidates 1.83 -> 1.68 ms at 7 waves; spatial
-// reuse + shading 0.537 ms at 6, 0.525 ms at 7).
+// reuse + shading 0.537 ms at 6, 0.525 ms at 7; with the MIS merge 2.09 ms
+// at 5, 2.29 ms at 7).
 #define VR_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
 // One-view stage kernels, instantiated for occupancy targets of 5-8 waves
@@ -667,17 +762,29 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
 // longest of its few: the tile's ray pixels are gathered in LDS (in pixel
 // order) and shaded by the first threads, so they fill as few waves as
 // possible and the other waves finish at once.  Uniform control flow: every
-// thread reaches both barriers.
+// thread reaches every barrier (the MIS weight's included).
 template <bool kMis>
 __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
 {
     __shared__ uint32_t rayPx[256];  // threads of the tile whose pixel needs a ray
     __shared__ uint32_t waveRays[4];
-    bool defer = false;
-    if (x < (int)a.W && y < (int)a.H) {
-        spatial_px<kMis>(a, x, y);
-        defer = shade_px<true>(a, x, y);
+    const bool in = x < (int)a.W && y < (int)a.H;
+    if (kMis) {
+        Merge g{};
+        if (in) g = spatial_merge<true>(a, x, y);
+        const float W = mis_weight_tile(a, x, y, g, in);
+        if (in) {
+            const size_t idx = (size_t)y * a.W + x;
+            if (g.merged) {
+                g.res.W = W;
+                a.cur.seed[idx] = g.seed;
+            }
+            store_res(a.A, idx, g.res);
+        }
+    } else if (in) {
+        spatial_px(a, x, y);
     }
+    const bool defer = in && shade_px<true>(a, x, y);
     const uint64_t m = __ballot(defer);
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     if (lane == 0u) waveRays[wv] = (uint32_t)__popcll(m);
@@ -721,7 +828,7 @@ __global__ __launch_bounds__(256) VR_WAVES(7) void k_initial_views(ViewChunk c)
 // kAnyMis: some view merges with MIS weights (the kernel then carries the
 // MIS variant's registers)
 template <bool kPersistent, bool kAnyMis>
-__global__ __launch_bounds__(256) VR_WAVES(7) void k_resolve_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(kAnyMis ? kDefaultWavesMis : kDefaultWaves[2]) void k_resolve_views(ViewChunk c)
 {
     many_views<kPersistent, true>(c, [](const FrameArgs& v, int x, int y) {
         if (kAnyMis && v.misSpatial) resolve_tile<true>(v, x, y);
@@ -863,7 +970,7 @@ static const void* stage_kernel_w(int stage, bool mis)
 // waves 0 = the stage's default target
 static const void* stage_kernel(int stage, bool mis, uint32_t waves)
 {
-    if (waves == 0) waves = kDefaultWaves[stage < 0 || stage > 2 ? 0 : stage];
+    if (waves == 0) waves = stage == 2 && mis ? kDefaultWavesMis : kDefaultWaves[stage < 0 || stage > 2 ? 0 : stage];
     switch (waves) {
     case 5: return stage_kernel_w<5>(stage, mis);
     case 6: return stage_kernel_w<6>(stage, mis);
