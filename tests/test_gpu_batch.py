@@ -14,12 +14,13 @@ pytestmark = pytest.mark.gpu
 
 def _instances():
     """Three small views of different worlds, sizes and methods: point
-    lights; area lights with the MIS spatial merge; the learned proposal."""
+    lights; area lights with the MIS spatial merge over 9 taps (two of the
+    tile's ray lists); the learned proposal."""
     c1 = configs.CONFIGS["c1_point_128"]
     c2 = configs.CONFIGS["c2_area_1024"]
     return [
         dataclasses.replace(c1, width=96, height=64, capacity=1 << 14),
-        dataclasses.replace(c2, width=120, height=72, flags=_lib.FLAG_MIS_SPATIAL, capacity=1 << 19),
+        dataclasses.replace(c2, width=120, height=72, flags=_lib.FLAG_MIS_SPATIAL, capacity=1 << 19, taps=9),
         dataclasses.replace(c1, width=40, height=24, point=False, source=_lib.SOURCE_LEARNED, capacity=1 << 14),
     ]
 

@@ -98,6 +98,19 @@ def test_mis_spatial_merge_matches_oracle(gpu, name, frames):
     assert all(c.ok for c in res), report
 
 
+@pytest.mark.parametrize("taps,radius", [(12, 30), (32, 8)])
+def test_mis_spatial_merge_many_taps(gpu, taps, radius):
+    """More combined taps than one of the tile's ray lists takes (5 per
+    pixel): the MIS weight's visibility rays run over several lists, in tap
+    order, up to the 32-tap limit."""
+    import dataclasses
+    sc = dataclasses.replace(configs.CONFIGS["c2_area_1024"], taps=taps, radius=radius)
+    res = run_config(sc, frames=3, width=160, height=96, flags=_lib.FLAG_MIS_SPATIAL)
+    report = "\n".join(c.report() for c in res)
+    print(report)
+    assert all(c.ok for c in res), report
+
+
 def test_mis_spatial_merge_rejects_too_many_taps(gpu):
     sc = configs.CONFIGS["c1_point_128"]
     import dataclasses
