@@ -139,7 +139,8 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
     const float hs = 0.5f * size;
     const float area = size * size;
     float sol[3];
-    int fidx[3];  // front face of each axis, or -1
+    int fidx[3];   // front face of each axis, or -1
+    f3 fp0, fp1, fp2;  // the point drawn on that face (named: an array would be indexed at run time below)
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float ca = a == 0 ? c.x : (a == 1 ? c.y : c.z);
@@ -150,7 +151,11 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
         const int i = plus ? a : a + 3;
         const float r0 = (float)(plus ? u[2 * a] : u[2 * a + 6]) * 2.3283064365387e-10f;
         const float r1 = (float)(plus ? u[2 * a + 1] : u[2 * a + 7]) * 2.3283064365387e-10f;
-        const f3 R = sub(face_point(c, hs, size, a, s, r0, r1), sp);
+        const f3 fpa = face_point(c, hs, size, a, s, r0, r1);
+        if (a == 0) fp0 = fpa;
+        else if (a == 1) fp1 = fpa;
+        else fp2 = fpa;
+        const f3 R = sub(fpa, sp);
         const float Ra = plus ? Rp : Rm;
         const float d = dot(R, R);
         const float inv = 1.0f / sqrtf(d);
@@ -192,13 +197,14 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
     if (isinf(wgt) || isnan(wgt)) wgt = 0.0f;
     invPdf = wgt;
     face = (uint32_t)index;
+    // With a non-zero weight the chosen face is its axis's front face, whose
+    // point the loop drew from the same two numbers.  With weight 0 the point
+    // never reaches an output (p-hat is finite, times 0; the reservoir
+    // cannot take a zero weight), so that axis's point serves as well.
     const int a = index % 3;
-    const float s = index < 3 ? 1.0f : -1.0f;
-    uint32_t w0 = u[0], w1 = u[1];
-#pragma unroll
-    for (int k = 1; k < 6; k++)
-        if (index == k) w0 = u[2 * k], w1 = u[2 * k + 1];
-    pos = face_point(c, hs, size, a, s, (float)w0 * 2.3283064365387e-10f, (float)w1 * 2.3283064365387e-10f);
+    pos.x = a == 0 ? fp0.x : (a == 1 ? fp1.x : fp2.x);
+    pos.y = a == 0 ? fp0.y : (a == 1 ? fp1.y : fp2.y);
+    pos.z = a == 0 ? fp0.z : (a == 1 ? fp1.z : fp2.z);
 }
 
 __device__ __forceinline__ uint32_t pick_light(const LightView& L, uint32_t source, uint32_t& seed, float& p)
