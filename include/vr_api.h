@@ -240,7 +240,10 @@ int vr_read_reservoirs(vr_ctx* ctx, int which, float* a4, float* b4, uint32_t* m
 int vr_read_frame(vr_ctx* ctx, float* rgba);
 
 /* Per-stage device time of the last rendered frame, milliseconds, measured
- * with HIP events on the render stream (template/world.cpp:2439-2471). */
+ * with HIP events around each stage (template/world.cpp:2439-2471).  When
+ * nothing changed the world since the previous frame's candidate pass, the
+ * primary pass runs on a second stream beside that frame's last stage; its
+ * time then spans its run next to that stage. */
 int vr_stage_times(vr_ctx* ctx, float ms[VR_NUM_STAGES]);
 
 /* Stage profiling over many frames (the reference keeps 100-frame averages,

@@ -25,8 +25,17 @@ struct vr_ctx {
     vr_config cfg;
     int device = 0;
     hipStream_t stream = nullptr;
-    // stage boundary events of the last kRing frames
-    hipEvent_t ev[kRing][VR_NUM_STAGES + 1] = {};
+    // The next frame's primary pass may run on `side` beside this frame's
+    // spatial reuse + shading: it writes the hit buffer that only this
+    // frame's candidate pass read (`initDone`), unless work queued since then
+    // changed the world it traces (`worldEpoch`, advanced by vr_scene_step;
+    // vr_commit returns only once its uploads are done).
+    hipStream_t side = nullptr;
+    hipEvent_t initDone = nullptr, primDone = nullptr;
+    bool initRecorded = false;
+    uint64_t worldEpoch = 0, epochAtInit = 0;
+    // start and end events of each stage of the last kRing frames
+    hipEvent_t ev[kRing][2 * VR_NUM_STAGES] = {};
     bool pending[kRing] = {};
     uint64_t frameNo = 0;
     int lastSlot = -1;
@@ -240,10 +249,10 @@ int harvest(vr_ctx* c, int slot)
 {
     if (!c->pending[slot]) return VR_OK;
     c->pending[slot] = false;
-    HIPCHK(c, hipEventSynchronize(c->ev[slot][VR_NUM_STAGES]));
+    HIPCHK(c, hipEventSynchronize(c->ev[slot][2 * VR_NUM_STAGES - 1]));
     for (int s = 0; s < VR_NUM_STAGES; s++) {
         float ms = 0.0f;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[slot][s], c->ev[slot][s + 1]));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[slot][2 * s], c->ev[slot][2 * s + 1]));
         c->profSum[s] += ms;
     }
     c->profFrames++;
@@ -430,9 +439,12 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     c->cfg = *cfg;
     const size_t n = (size_t)cfg->width * cfg->height;
     auto ok = [&](hipError_t e) { return e == hipSuccess; };
-    bool good = ok(hipGetDevice(&c->device)) && ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    bool good = ok(hipGetDevice(&c->device)) && ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) &&
+                ok(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) &&
+                ok(hipEventCreateWithFlags(&c->initDone, hipEventDisableTiming)) &&
+                ok(hipEventCreateWithFlags(&c->primDone, hipEventDisableTiming));
     for (int f = 0; good && f < kRing; f++)
-        for (int i = 0; good && i <= VR_NUM_STAGES; i++) good = ok(hipEventCreate(&c->ev[f][i]));
+        for (int i = 0; good && i < 2 * VR_NUM_STAGES; i++) good = ok(hipEventCreate(&c->ev[f][i]));
     good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 1024)) &&
            ok(dalloc(&c->bricks, (size_t)cfg->brick_capacity * VR_BRICKSIZE)) &&
            ok(dalloc(&c->stageIds, VR_MAXCOMMITS)) && ok(dalloc(&c->stageData, (size_t)VR_MAXCOMMITS * VR_BRICKSIZE)) &&
@@ -479,9 +491,12 @@ int vr_destroy(vr_ctx* c)
     dfree(c->candIdx), dfree(c->candPdf);
     dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
     dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget), dfree(c->claim);
-    for (int i = 0; i <= VR_NUM_STAGES; i++)
+    for (int i = 0; i < 2 * VR_NUM_STAGES; i++)
         for (int f = 0; f < kRing; f++)
             if (c->ev[f][i]) (void)hipEventDestroy(c->ev[f][i]);
+    if (c->initDone) (void)hipEventDestroy(c->initDone);
+    if (c->primDone) (void)hipEventDestroy(c->primDone);
+    if (c->side) (void)hipStreamSynchronize(c->side), (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VR_OK;
@@ -598,18 +613,33 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         const int rc = harvest(c, slot);
         if (rc != VR_OK) return rc;
     }
-    HIPCHK(c, hipEventRecord(c->ev[slot][0], c->stream));
+    // the primary pass beside the previous frame's last stage when nothing
+    // since that frame's candidate pass has changed the world
+    const bool overlap = c->initRecorded && c->epochAtInit == c->worldEpoch;
+    hipStream_t ps = overlap ? c->side : c->stream;
+    if (overlap) HIPCHK(c, hipStreamWaitEvent(c->side, c->initDone, 0));
     for (int s = 0; s < VR_NUM_STAGES; s++) {
         FrameArgs as = a;
         as.tileOrder = (c->cfg.flags >> (2 * s)) & 3u;
         as.waves = c->stageWaves[s];
+        hipStream_t st = s == VR_STAGE_PRIMARY ? ps : c->stream;
+        if (s == VR_STAGE_INITIAL && overlap) {
+            HIPCHK(c, hipEventRecord(c->primDone, c->side));
+            HIPCHK(c, hipStreamWaitEvent(c->stream, c->primDone, 0));
+        }
+        HIPCHK(c, hipEventRecord(c->ev[slot][2 * s], st));
         // the proposal runs after the primary hits and is timed with the
         // candidate stage
         if (s == VR_STAGE_INITIAL && learned)
             HIPCHK(c, launch_proposal(as, mlp_view(c), c->candIdx, c->candPdf, proposal_salt(p->R0),
                                       c->stream));
-        HIPCHK(c, launch_stage(s, as, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev[slot][s + 1], c->stream));
+        HIPCHK(c, launch_stage(s, as, st));
+        HIPCHK(c, hipEventRecord(c->ev[slot][2 * s + 1], st));
+        if (s == VR_STAGE_INITIAL) {
+            HIPCHK(c, hipEventRecord(c->initDone, c->stream));
+            c->initRecorded = true;
+            c->epochAtInit = c->worldEpoch;
+        }
     }
     c->pending[slot] = c->profiling;
     c->lastSlot = slot;
@@ -682,9 +712,9 @@ int vr_stage_times(vr_ctx* c, float ms[VR_NUM_STAGES])
     if (c->lastSlot < 0)
         return fail(c, VR_ERR_STATE, "vr_stage_times: the last frame was rendered in a batch (vr_batch_profile_read)");
     const int slot = c->lastSlot;
-    HIPCHK(c, hipEventSynchronize(c->ev[slot][VR_NUM_STAGES]));
+    HIPCHK(c, hipEventSynchronize(c->ev[slot][2 * VR_NUM_STAGES - 1]));
     for (int s = 0; s < VR_NUM_STAGES; s++)
-        HIPCHK(c, hipEventElapsedTime(&ms[s], c->ev[slot][s], c->ev[slot][s + 1]));
+        HIPCHK(c, hipEventElapsedTime(&ms[s], c->ev[slot][2 * s], c->ev[slot][2 * s + 1]));
     return VR_OK;
 }
 
@@ -1053,6 +1083,7 @@ int vr_scene_step(vr_ctx* c, uint32_t frame, uint32_t seed, uint32_t edit_permil
         return fail(c, VR_ERR_ARG, "vr_scene_step: edit_permille <= 1000, walk_steps <= 4096 and high_water <= "
                                    "brick capacity");
     HIPCHK(c, hipSetDevice(c->device));
+    c->worldEpoch++;  // the next primary pass waits for these edits
     HIPCHK(c, launch_edit_bricks(c->bricks, high_water, frame, seed, edit_permille, walk_steps, c->stream));
     if (c->motionN > 0 && c->motionN == c->nLights) {
         const LightMotionView m{c->la, c->lb, c->lightPos, c->lightVel, c->lightVoxel, c->lightTarget, c->claim,
@@ -1334,6 +1365,7 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
         }
         c->lastCam = a.cam;
         c->lastSlot = -1;
+        c->initRecorded = false;  // this frame's passes run on the batch's streams
         c->frameNo++;
         c->framed = true;
         // uploads queued on the view's own stream (commits, lights) first
