@@ -12,8 +12,8 @@
  * throws; vr_last_error() describes the last failure of a context.  Pointers
  * are host pointers unless a name says otherwise.  A context is not
  * thread-safe: one host thread drives it (as the reference's single render
- * thread does).  Rendering is asynchronous on the context's HIP stream;
- * read-backs and vr_sync() block.
+ * thread does).  Rendering is asynchronous on the context's two HIP streams
+ * (see vr_stage_times); read-backs and vr_sync() block.
  */
 #ifndef VR_API_H
 #define VR_API_H
