@@ -178,3 +178,34 @@ def test_occupancy_targets_render_identical_frames(gpu):
     with pytest.raises(_lib.VrError):
         r2 = configs.build_world(sc, 32, 32).renderer
         r2.set_stage_waves(9)
+
+
+@pytest.mark.parametrize("name,flags", [("c2_area_1024", 0), ("c2_area_1024", _lib.FLAG_MIS_SPATIAL)])
+def test_back_to_back_frames_equal_synchronised_frames(gpu, name, flags):
+    """Frames queued back to back let each frame's primary pass run on the
+    context's second stream beside the previous frame's last stage.  The
+    same frames, each waited for before the next is queued, must come out
+    bit for bit the same (hits, both reservoir sets, radiance)."""
+    import dataclasses
+    sc = dataclasses.replace(configs.CONFIGS[name], flags=flags)
+    outs = []
+    for wait in (False, True):
+        w = configs.build_world(sc, 320, 180)
+        w.force_sync()
+        r = w.renderer
+        for f in range(6):
+            pos, tgt = sc.camera(f)
+            w.look_at(pos, tgt)
+            w.render()
+            if wait:
+                r.sync()
+        hw, d, sd = r.hits()
+        o = [hw, d.view(np.uint32), sd, r.frame().view(np.uint32)]
+        for which in (0, 1):
+            a, b, m = r.reservoirs(which)
+            o += [a.view(np.uint32), b.view(np.uint32), m]
+        outs.append(o)
+        w.close()
+    for k, (g, e) in enumerate(zip(*outs)):
+        assert np.array_equal(g, e), f"output {k}: {np.mean(g != e):.6f} differ"
+
