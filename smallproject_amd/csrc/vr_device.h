@@ -196,8 +196,9 @@ __device__ uint32_t trace(const WorldView& w, f3 A, f3 B, int steps, uint32_t th
     const uint32_t dx = V.x > 0 ? (1u << 20) : (0xfffu << 20);
     const uint32_t dy = V.y > 0 ? (1u << 10) : 0xfffffc00u;
     const uint32_t dz = V.z > 0 ? 1u : 0xffffffffu;
-    const f3 td = f3{(V.x > 0 ? 1.0f : -1.0f) * rV.x, (V.y > 0 ? 1.0f : -1.0f) * rV.y,
-                     (V.z > 0 ? 1.0f : -1.0f) * rV.z};
+    // cell crossing steps sign(V) * (1/V), which is |1/V| exactly (V != 0):
+    // the absolute value is a free source modifier, so no registers hold it
+    const f3 td = f3{fabsf(rV.x), fabsf(rV.y), fabsf(rV.z)};
     float to = 0.0f;
     uint32_t last = 0;
     if (A.x < 0 || A.y < 0 || A.z < 0 || A.x > 1024.0f || A.y > 1024.0f || A.z > 1024.0f) {
