@@ -80,9 +80,13 @@ __device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
 }
 
 // default occupancy targets (waves per SIMD) of the stage kernels: primary,
-// candidates, spatial reuse + shading; the MIS merge's variant of the last
-// holds its tile's ray list in LDS and more live values, and runs best at 5
-constexpr uint32_t kDefaultWaves[3] = {8, 7, 7};
+// candidates, spatial reuse + shading.  The primary pass mostly runs beside
+// the previous frame's last stage, where fewer of its waves slow that stage
+// less (c2: 0.24 ms at 5 against 0.31 ms at 8, the same when alone); the
+// last stage then runs best at 8 (0.57 against 0.62 ms at 7, the same when
+// alone).  The MIS merge's variant holds its tile's ray list in LDS and more
+// live values, and runs best at 5.
+constexpr uint32_t kDefaultWaves[3] = {5, 7, 8};
 constexpr uint32_t kDefaultWavesMis = 5;
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s);
 // Up to kChunkViews views rendered by one launch per stage: their

@@ -744,9 +744,8 @@ __device__ __forceinline__ void many_views(const ViewChunk& c, F f)
 // Occupancy targets (waves per SIMD) of the ray-tracing stages: their DDA
 // loops are bound by dependent-load latency, and more waves in flight hide
 // more of it than the few registers the compiler then spills cost
-// (measured on c2 at 1080p: candidates 1.83 -> 1.68 ms at 7 waves; spatial
-// reuse + shading 0.537 ms at 6, 0.525 ms at 7; with the MIS merge 2.09 ms
-// at 5, 2.29 ms at 7).
+// (measured on c2 at 1080p: candidates 1.83 -> 1.68 ms at 7 waves; the
+// defaults of the other stages are explained where they are set).
 #define VR_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
 // One-view stage kernels, instantiated for occupancy targets of 5-8 waves
@@ -832,9 +831,10 @@ __global__ __launch_bounds__(256) VR_WAVES(7) void k_initial_views(ViewChunk c)
     many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
 }
 // kAnyMis: some view merges with MIS weights (the kernel then carries the
-// MIS variant's registers)
+// MIS variant's registers).  Batch frames do not overlap one another, so the
+// reference merge's kernel keeps the 7 waves it runs best at alone.
 template <bool kPersistent, bool kAnyMis>
-__global__ __launch_bounds__(256) VR_WAVES(kAnyMis ? kDefaultWavesMis : kDefaultWaves[2]) void k_resolve_views(ViewChunk c)
+__global__ __launch_bounds__(256) VR_WAVES(kAnyMis ? kDefaultWavesMis : 7) void k_resolve_views(ViewChunk c)
 {
     many_views<kPersistent, true>(c, [](const FrameArgs& v, int x, int y) {
         if (kAnyMis && v.misSpatial) resolve_tile<true>(v, x, y);
