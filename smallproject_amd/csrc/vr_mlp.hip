@@ -22,13 +22,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ __bf16 to_bf16(float f)
 {
-    // round to nearest even (finite inputs)
-    uint32_t u = __float_as_uint(f);
-    u = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-    const unsigned short s = (unsigned short)u;
-    __bf16 b;
-    __builtin_memcpy(&b, &s, 2);
-    return b;
+    // round to nearest even: gfx950's v_cvt_pk_bf16_f32
+    return static_cast<__bf16>(f);
 }
 
 __device__ __forceinline__ float from_bf16(__bf16 b)
@@ -329,7 +324,10 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
         };
         // slots [kBeg, kEnd) belong to this lane: the lower lane takes the
         // targets below massLo
-        int kSplit = 0;
+        // target(j) = (j + u_j)/32 with u_j <= 1 (as rounded), so every j
+        // below floor(32 massLo) - 2 lies strictly under massLo: the search
+        // starts there and finds the same first target at or above it
+        int kSplit = (int)fminf(fmaxf(floorf(massLo * 32.0f) - 2.0f, 0.0f), 32.0f);
         while (kSplit < 32 && target(kSplit) < massLo) kSplit++;
         const bool upper = lane >= 32;
         const int kBeg = upper ? kSplit : 0, kEnd = upper ? 32 : kSplit;
