@@ -111,6 +111,19 @@ def test_mis_spatial_merge_many_taps(gpu, taps, radius):
     assert all(c.ok for c in res), report
 
 
+@pytest.mark.parametrize("flags", [0, _lib.FLAG_MIS_SPATIAL], ids=["biased", "mis"])
+def test_world_without_lights_matches_oracle(gpu, flags):
+    """A world with no emitters: every candidate, reuse and shading guard on
+    the light index is exercised (nothing may be read from an empty light
+    list), and the frame is the oracle's bit for bit."""
+    import dataclasses
+    sc = dataclasses.replace(configs.CONFIGS["c2_area_1024"], lights=0)
+    res = run_config(sc, frames=3, width=160, height=96, flags=flags)
+    report = "\n".join(c.report() for c in res)
+    print(report)
+    assert all(c.ok for c in res), report
+
+
 def test_mis_spatial_merge_rejects_too_many_taps(gpu):
     sc = configs.CONFIGS["c1_point_128"]
     import dataclasses
