@@ -124,6 +124,16 @@ def test_world_without_lights_matches_oracle(gpu, flags):
     assert all(c.ok for c in res), report
 
 
+@pytest.mark.parametrize("flags", [0, _lib.FLAG_MIS_SPATIAL], ids=["biased", "mis"])
+def test_small_odd_resolution_matches_oracle(gpu, flags):
+    """37x23 pixels: every 16x16 tile is partial at the right or bottom
+    edge and most spatial taps fall outside the image."""
+    res = run_config(configs.CONFIGS["c2_area_1024"], frames=3, width=37, height=23, flags=flags)
+    report = "\n".join(c.report() for c in res)
+    print(report)
+    assert all(c.ok for c in res), report
+
+
 def test_mis_spatial_merge_rejects_too_many_taps(gpu):
     sc = configs.CONFIGS["c1_point_128"]
     import dataclasses
