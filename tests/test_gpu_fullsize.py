@@ -1,9 +1,10 @@
 """The benchmark workloads at their full size (1920x1080, the 1024^3 worlds)
-against the oracle, bit for bit on every output plane: hit words,
-distances, seeds, the candidate-stage and final reservoirs (every float
-field included) and the radiance.  The reduced-resolution parity tests
-cover more frames and settings; these pin the exact sizes the benchmark
-measures, including the tile grid's edge (1080 = 67.5 tiles of 16 rows)."""
+and over their whole camera paths, against the oracle, bit for bit on every
+output plane: hit words, distances, seeds, the candidate-stage and final
+reservoirs (every float field included) and the radiance.  The full-size
+cases pin the exact sizes the benchmark measures, including the tile
+grid's edge (1080 = 67.5 tiles of 16 rows); the path cases carry temporal
+reuse over many frames."""
 import dataclasses
 
 import numpy as np
@@ -62,3 +63,33 @@ def test_c3_learned_full_size_bit_exact(gpu):
     sc = dataclasses.replace(configs.CONFIGS["c3_clusters_1024"], source=_lib.SOURCE_LEARNED)
     res = run_config(sc, frames=2, setup=setup, strict=True)
     _check(res, "c3 learned")
+
+
+def test_c2_whole_camera_path_bit_exact(gpu):
+    """All 60 frames of c2's spline camera path at 320x180: temporal reuse
+    carries 60 frames of history, and every frame stays bit-exact."""
+    sc = configs.CONFIGS["c2_area_1024"]
+    res = run_config(sc, frames=sc.frames, width=320, height=180)
+    for f, cmp in enumerate(res):
+        assert cmp.ok, f"frame {f}\n{cmp.report()}"
+
+
+def test_c5_moving_scene_24_frames_bit_exact(gpu):
+    """24 frames of c5's moving scene at 320x180 (emitters moving, bricks
+    edited on the device before each frame, MIS merge)."""
+    sc = configs.CONFIGS["c5_dynamic_1024"]
+    state = {}
+
+    def before_frame(f, w, r, view, of):
+        if f == 0:
+            configs.start_motion(sc, w)
+            state["m"] = O.Motion(view, of.lights, configs.light_velocities(len(of.lights), sc.light_speed,
+                                                                              sc.motion_seed))
+            state["hw"] = w.high_water()
+        configs.scene_step(sc, w, f)
+        state["m"].step(f, sc.edit_seed, sc.edit_permille, sc.walk_steps, state["hw"])
+
+    res = run_config(sc, frames=24, width=320, height=180, before_frame=before_frame)
+    for f, cmp in enumerate(res):
+        assert cmp.ok, f"frame {f}\n{cmp.report()}"
+
