@@ -12,8 +12,9 @@
  * throws; vr_last_error() describes the last failure of a context.  Pointers
  * are host pointers unless a name says otherwise.  A context is not
  * thread-safe: one host thread drives it (as the reference's single render
- * thread does).  Rendering is asynchronous on the context's two HIP streams
- * (see vr_stage_times); read-backs and vr_sync() block.
+ * thread does).  Rendering is asynchronous on the context's HIP stream (two
+ * while it is the only context: see vr_stage_times); read-backs and vr_sync()
+ * block.
  */
 #ifndef VR_API_H
 #define VR_API_H
@@ -241,9 +242,10 @@ int vr_read_frame(vr_ctx* ctx, float* rgba);
 
 /* Per-stage device time of the last rendered frame, milliseconds, measured
  * with HIP events around each stage (template/world.cpp:2439-2471).  When
- * nothing changed the world since the previous frame's candidate pass, the
- * primary pass runs on a second stream beside that frame's last stage; its
- * time then spans its run next to that stage. */
+ * the context is the only one in the process and nothing changed the world
+ * since the previous frame's candidate pass, the primary pass runs on a
+ * second stream beside that frame's last stage; its time then spans its run
+ * next to that stage. */
 int vr_stage_times(vr_ctx* ctx, float ms[VR_NUM_STAGES]);
 
 /* Stage profiling over many frames (the reference keeps 100-frame averages,
