@@ -157,7 +157,7 @@ def bench_batch(args) -> int:
     frames.  Also times the same frames rendered view after view (one
     launch per stage and view) for comparison."""
     from smallproject_amd import Batch, configs
-    scs = configs.batch_instances()
+    scs = configs.small_instances() if args.config == "c4_small64" else configs.batch_instances()
     t0 = time.perf_counter()
     worlds = [configs.build_world(sc) for sc in scs]
     for w in worlds:
@@ -265,7 +265,7 @@ def main() -> int:
         return 2
     if args.calibrate:
         return calibrate()
-    if args.config == "c4_batch16":
+    if args.config in ("c4_batch16", "c4_small64"):
         return bench_batch(args)
 
     import dataclasses

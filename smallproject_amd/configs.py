@@ -20,6 +20,8 @@ and render settings.  `build_world` constructs the host World (CPU only);
   batch_instances()  16 scene instances of uneven world size (256^3 -
                  1024^3) and resolution (320x180 - 1920x1080) rendered as one
                  batch of views
+  small_instances()  64 instances of 320x180 on 256^3 worlds: many small
+                 views, where one launch per stage for all of them matters
 """
 from __future__ import annotations
 
@@ -132,6 +134,25 @@ def batch_instances(n: int = 16, seed: int = 8, frames: int = 30) -> list[SceneC
             box_edge=(4, 32), box_seed=base + 1, lights=lights, light_seed=base + 2, light_mode="surface",
             emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=base + 3,
             capacity={256: 1 << 15, 512: 1 << 17, 1024: 1 << 19}[size]))
+    return out
+
+
+def small_instances(n: int = 64, seed: int = 18, frames: int = 30) -> list[SceneConfig]:
+    """Many small views: n instances of 320x180 on 256^3 worlds (heightfield
+    scale 2, 125 boxes), 256-4,096 emissive surface voxels (log-uniform
+    count), area lights, each on its own camera path; numpy PCG64 seeded with
+    `seed`."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        lights = int(round(math.exp(rng.uniform(math.log(256), math.log(4096)))))
+        base = int(rng.integers(1, 1 << 30))
+        out.append(SceneConfig(
+            name=f"c4s_view{i}", width=320, height=180, point=False, fov=60.0, frames=frames, world_size=256,
+            scale=2, terrain_seed=base, boxes=125, box_edge=(4, 32), box_seed=base + 1, lights=lights,
+            light_seed=base + 2, light_mode="surface", emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1,
+            camera_seed=base + 3, capacity=1 << 15))
     return out
 
 
