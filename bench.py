@@ -109,12 +109,25 @@ def quality(log_fn) -> dict:
     """The metric's image-quality half: equal-time MSE against a 4096-spp
     binary64 converged reference, learned proposal vs the plain power
     source, on the many-light configuration at 1920x1080, with the
-    reference's (biased) spatial merge and with the MIS-weighted one."""
+    reference's (biased) spatial merge and with the MIS-weighted one; the
+    dynamic-scene run; and the plain method's MSE on the point-light and
+    area-light configurations."""
     from smallproject_amd import _lib
-    from smallproject_amd.evaluate import dynamic
+    from smallproject_amd.evaluate import dynamic, plain_mse
     out = {}
     for merge, flags in (("biased", 0), ("mis", _lib.FLAG_MIS_SPATIAL)):
         out[merge] = quality_one(log_fn, merge, flags)
+    # the plain method's own MSE on the point-light and area-light
+    # configurations (8 frames from reset history at the middle camera)
+    out["plain"] = {}
+    for name in ("c1_point_128", "c2_area_1024"):
+        t0 = time.perf_counter()
+        r = plain_mse(name)
+        log_fn(f"bench: {name} MSE against the converged reference in {time.perf_counter() - t0:.1f} s")
+        out["plain"][name] = {
+            "reference": f"{r['gt_spp']}-spp binary64 ground truth (its own noise MSE {r['gt_noise_mse']:.2e})",
+            "resolution": r["resolution"], "camera_frame": r["camera_frame"],
+            "mse_per_frame": [round(v, 6) for v in r["mse_per_frame"]], "ms_per_frame": round(r["ms_per_frame"], 4)}
     t0 = time.perf_counter()
     d = dynamic("c5_dynamic_1024")
     log_fn(f"bench: dynamic-scene learned/plain run in {time.perf_counter() - t0:.1f} s")
