@@ -38,7 +38,9 @@ HBM_PEAK_GBS = 8000.0
 #   resolve (spatial reuse + shading, one kernel): read own reservoir 36
 #            + hit 12; write reservoir 36, seed 4, frame 16             104
 STAGE_BYTES = {"primary": 12, "initial": 96, "resolve": 104}
-RAYS_PER_PIXEL = 3  # primary + candidate shadow ray + shading shadow ray
+# rays per pixel as the reference traces them: primary, candidate shadow ray,
+# shading shadow ray (this path looks most shading rays up instead)
+RAYS_PER_PIXEL = 3
 
 
 def log(msg: str) -> None:
