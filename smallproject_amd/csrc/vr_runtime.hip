@@ -563,7 +563,10 @@ int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
         const uint32_t cap = n * 2;
         if (dalloc(&c->la, cap) != hipSuccess || dalloc(&c->lb, cap) != hipSuccess || dalloc(&c->prob, cap) != hipSuccess ||
             dalloc(&c->alias, cap) != hipSuccess || dalloc(&c->pmf, cap) != hipSuccess) {
-            c->lightCap = 0;
+            // the old list's buffers are gone: leave an empty list, so that
+            // no later frame or step can reach the freed records
+            c->lightCap = 0, c->nLights = 0, c->motionN = 0;
+            c->hostLights.clear();
             return fail(c, VR_ERR_NOMEM, "vr_set_lights: device allocation failed");
         }
         c->lightCap = cap;
