@@ -353,7 +353,14 @@ int mlp_alloc(vr_ctx* c)
                       ok(dalloc(&c->mlpW3Tb, 256 * 64)) && ok(dalloc(&c->clusterStart, kMaxClusters + 1)) &&
                       ok(dalloc(&c->clusterPower, kMaxClusters)) &&
                       ok(dalloc(&c->candIdx, npx * kCandidates)) && ok(dalloc(&c->candPdf, npx * kCandidates));
-    if (!good) return fail(c, VR_ERR_NOMEM, "learned proposal: device allocation failed");
+    if (!good) {
+        // free what was allocated, so that a later call starts clean
+        for (void** q : {(void**)&c->mlpP, (void**)&c->mlpM, (void**)&c->mlpV, (void**)&c->mlpG, (void**)&c->mlpW1b,
+                         (void**)&c->mlpW2b, (void**)&c->mlpW2Tb, (void**)&c->mlpW3b, (void**)&c->mlpW3Tb,
+                         (void**)&c->clusterStart, (void**)&c->clusterPower, (void**)&c->candIdx, (void**)&c->candPdf})
+            dfree(*q), *q = nullptr;
+        return fail(c, VR_ERR_NOMEM, "learned proposal: device allocation failed");
+    }
     // He-uniform weights (bound sqrt(6/fan_in)), zero biases; the output
     // layer is scaled by 1/8 so the first proposals are near-uniform
     std::vector<float> init(kMlpParams, 0.0f);
@@ -839,6 +846,8 @@ int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights
         dfree(c->clusterLights), dfree(c->clusterOf), dfree(c->clusterProb), dfree(c->clusterAlias), dfree(c->lightShare);
         c->clusterLights = nullptr, c->clusterOf = nullptr, c->clusterProb = nullptr, c->clusterAlias = nullptr,
         c->lightShare = nullptr, c->clusterLightCap = 0;
+        // the old tables are gone: no frame may use them if this fails
+        c->nclusters = 0, c->clusteredLights = 0;
         if (dalloc(&c->clusterLights, nlights) != hipSuccess || dalloc(&c->clusterOf, nlights) != hipSuccess ||
             dalloc(&c->clusterProb, nlights) != hipSuccess || dalloc(&c->clusterAlias, nlights) != hipSuccess ||
             dalloc(&c->lightShare, nlights) != hipSuccess)
@@ -1076,6 +1085,7 @@ int vr_set_light_motion(vr_ctx* c, const float* velocity, uint32_t n)
         dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget);
         c->lightPos = c->lightVel = nullptr, c->lightVoxel = c->lightTarget = nullptr;
         c->motionCap = 0;
+        c->motionN = 0;  // no scene step may move lights through the freed buffers
         if (dalloc(&c->lightPos, n) != hipSuccess || dalloc(&c->lightVel, n) != hipSuccess ||
             dalloc(&c->lightVoxel, n) != hipSuccess || dalloc(&c->lightTarget, n) != hipSuccess)
             return fail(c, VR_ERR_NOMEM, "vr_set_light_motion: device allocation failed");
