@@ -496,6 +496,7 @@ int vr_destroy(vr_ctx* c)
     if (!c) return VR_ERR_ARG;
     if (c->counted) g_liveContexts--;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     dfree(c->grid), dfree(c->uber), dfree(c->bricks), dfree(c->stageIds), dfree(c->stageData);
     dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf), dfree(c->emitLut), dfree(c->frame);
     dfree(c->nbr), dfree(c->tested);
@@ -515,7 +516,7 @@ int vr_destroy(vr_ctx* c)
             if (c->ev[f][i]) (void)hipEventDestroy(c->ev[f][i]);
     if (c->initDone) (void)hipEventDestroy(c->initDone);
     if (c->primDone) (void)hipEventDestroy(c->primDone);
-    if (c->side) (void)hipStreamSynchronize(c->side), (void)hipStreamDestroy(c->side);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VR_OK;
@@ -1314,6 +1315,8 @@ int vr_batch_destroy(vr_batch* b)
 {
     if (!b) return VR_ERR_ARG;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (int k = 0; k < vr_batch::kChainStreams; k++)
+        if (b->chainStream[k]) (void)hipStreamSynchronize(b->chainStream[k]);
     for (vr_batch::Slot& sl : b->slot) {
         drop_graph(sl);
         if (sl.begin) (void)hipEventDestroy(sl.begin);
@@ -1322,7 +1325,7 @@ int vr_batch_destroy(vr_batch* b)
     if (b->viewReady) (void)hipEventDestroy(b->viewReady);
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (int k = 0; k < vr_batch::kChainStreams; k++) {
-        if (b->chainStream[k]) (void)hipStreamSynchronize(b->chainStream[k]), (void)hipStreamDestroy(b->chainStream[k]);
+        if (b->chainStream[k]) (void)hipStreamDestroy(b->chainStream[k]);
         if (b->join[k]) (void)hipEventDestroy(b->join[k]);
     }
     if (b->stream) (void)hipStreamDestroy(b->stream);
