@@ -64,7 +64,9 @@ def _render_single(scs, frames):
                                                 (True, 0, 2)])
 def test_batch_renders_each_views_own_frames(gpu, graph, blocks, chunk):
     scs = _instances()
-    frames = 3
+    # a batch keeps 4 frames in flight, each captured once: with the graph
+    # on, frames 5-8 replay those captures with new kernel arguments
+    frames = 8
     want = _render_single(scs, frames)
     worlds = [_world(sc) for sc in scs]
     b = Batch(worlds)
