@@ -134,6 +134,19 @@ def test_small_odd_resolution_matches_oracle(gpu, flags):
     assert all(c.ok for c in res), report
 
 
+@pytest.mark.parametrize("flags", [0, _lib.FLAG_MIS_SPATIAL], ids=["biased", "mis"])
+@pytest.mark.parametrize("size", [(3840, 2160), (4096, 8)], ids=["3840x2160", "4096x8"])
+def test_large_and_strip_resolutions_match_oracle(gpu, size, flags):
+    """Beyond the benchmark's sizes: 3840x2160 (8.3 M pixels, 32,400 tiles,
+    four times the 1080p grid the XCD-aware tile orders map) and a 4096x8
+    strip (one partial row of 256 tiles; most spatial taps land above or
+    below the image)."""
+    res = run_config(configs.CONFIGS["c1_point_128"], frames=2, width=size[0], height=size[1], flags=flags)
+    report = "\n".join(c.report() for c in res)
+    print(report)
+    assert all(c.ok for c in res), report
+
+
 def test_mis_spatial_merge_rejects_too_many_taps(gpu):
     sc = configs.CONFIGS["c1_point_128"]
     import dataclasses
