@@ -176,6 +176,12 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
     float pdf = 0.0f;
     if (total == 0.0f) {
         index = 0;  // 0/0: the walk stops at face 0 with pdf NaN
+    } else if (r2 == 0.0f && fidx[0] != 0) {
+        // cdf[0] = 0 >= r2 = 0: the walk stops at face 0 whatever its solid
+        // angle (tools.cl:549-551); a +x face with none gives pdf 0, weight 0
+        // (r2 is 0 only for a pixel stream of 0, since xorshift32 maps no
+        // non-zero state to 0)
+        index = 0;
     } else {
         // the two faces of an axis share its solid angle: one division each
         float q[3];

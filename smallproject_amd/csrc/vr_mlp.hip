@@ -246,12 +246,13 @@ __device__ __forceinline__ void softmax_infer(const MlpView& m, const float* sBi
 }
 
 // ---------------------------------------------------------------------------
-// k_proposal: per pixel, 32 stratified cluster draws from the defensive
-// mixture m(c | x) = (1 - eps) q(c | x) + eps P_c, where q is the network's
-// softmax over the non-empty clusters and P_c the cluster's share of the
-// total light power.  Candidate j's target is (j + u_j)/32 of the mixture
-// mass, u_j = hash(pixel stream, j): the strata together cover the
-// distribution once, so the RIS weight p-hat/p stays unbiased.  The lane
+// k_proposal: per pixel, n = min(lights, candidates) <= 32 stratified cluster
+// draws from the defensive mixture m(c | x) = (1 - eps) q(c | x) + eps P_c,
+// where q is the network's softmax over the non-empty clusters and P_c the
+// cluster's share of the total light power.  Candidate j's target is
+// (j + u_j)/n of the mixture mass, u_j = hash(pixel stream, j): the n strata
+// together cover the distribution once (the candidate pass reads exactly
+// slots 0..n-1), so the RIS weight p-hat/p stays unbiased.  The lane
 // pair of a pixel splits the CDF (the lower lane holds the clusters with
 // (c & 4) == 0) and each lane walks its part once in cluster order, writing
 // slot j as (cluster, m_c); the candidate pass draws the light inside the
@@ -304,6 +305,10 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
     forward_infer(m, s, xin);
     const float keep = 1.0f - m.mix;
     const size_t plane = (size_t)a.W * a.H;
+    // slots the candidate pass reads (the host rejects more than 32)
+    const int ns = (int)(a.L.n < a.nCand ? a.L.n : a.nCand);
+    const float invNs = ns > 0 ? 1.0f / (float)ns : 0.0f;
+    const float fns = (float)ns;
     for (int nb = 0; nb < 2; nb++) {
         const int p = nb * 32 + (lane & 31);
         int x, y;
@@ -319,18 +324,17 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
         // stays untouched); both lanes of the pair see the same u_j
         const uint32_t ubase = wang_hash((valid ? a.cur.seed[idx] : 0u) ^ salt);
         auto target = [&](int j) {
-            return ((float)j + (float)wang_hash(ubase + (uint32_t)j * 0x9e3779b9u) * 2.3283064365387e-10f) *
-                   (1.0f / 32.0f);
+            return ((float)j + (float)wang_hash(ubase + (uint32_t)j * 0x9e3779b9u) * 2.3283064365387e-10f) * invNs;
         };
         // slots [kBeg, kEnd) belong to this lane: the lower lane takes the
         // targets below massLo
-        // target(j) = (j + u_j)/32 with u_j <= 1 (as rounded), so every j
-        // below floor(32 massLo) - 2 lies strictly under massLo: the search
+        // target(j) = (j + u_j)/n with u_j <= 1 (as rounded), so every j
+        // below floor(n massLo) - 2 lies strictly under massLo: the search
         // starts there and finds the same first target at or above it
-        int kSplit = (int)fminf(fmaxf(floorf(massLo * 32.0f) - 2.0f, 0.0f), 32.0f);
-        while (kSplit < 32 && target(kSplit) < massLo) kSplit++;
+        int kSplit = (int)fminf(fmaxf(floorf(massLo * fns) - 2.0f, 0.0f), fns);
+        while (kSplit < ns && target(kSplit) < massLo) kSplit++;
         const bool upper = lane >= 32;
-        const int kBeg = upper ? kSplit : 0, kEnd = upper ? 32 : kSplit;
+        const int kBeg = upper ? kSplit : 0, kEnd = upper ? ns : kSplit;
         const float base = upper ? massLo : 0.0f;
         int k = kBeg;
         float tk = k < kEnd ? target(k) : 2.0f;  // next target of this lane

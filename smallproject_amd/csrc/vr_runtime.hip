@@ -582,6 +582,8 @@ int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
     c->nLights = n;
     c->hostLights.assign(lights, lights + n);
     c->motionN = 0;  // a new list has no motion until vr_set_light_motion
+    // the cluster tables describe the previous list, even at the same size
+    c->clusteredLights = 0;
     if (n == 0) return VR_OK;
     HIPCHK(c, hipMemcpyAsync(c->la, ha.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->lb, hb.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
@@ -629,13 +631,14 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         if (rc != VR_OK) return rc;
     }
     const bool learned = c->cfg.source_pdf == VR_SOURCE_LEARNED;
-    c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
-    const FrameArgs a = frame_args(c, p);
     const int slot = (int)(c->frameNo % kRing);
     {
+        // before any state changes: a failure here leaves the context as it was
         const int rc = harvest(c, slot);
         if (rc != VR_OK) return rc;
     }
+    c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
+    const FrameArgs a = frame_args(c, p);
     // the primary pass beside the previous frame's last stage when nothing
     // since that frame's candidate pass has changed the world
     bool overlap = c->initRecorded && c->epochAtInit == c->worldEpoch && g_liveContexts.load() == 1;
@@ -1369,8 +1372,8 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
     const uint32_t n = (uint32_t)b->views.size();
     for (uint32_t i = 0; i < n; i++) {
         vr_ctx* c = b->views[i];
-        if (check_frame(c, &params[i], "vr_batch_render") != VR_OK)
-            return bfail(b, VR_ERR_ARG, "view " + std::to_string(i) + ": " + c->err);
+        const int rc = check_frame(c, &params[i], "vr_batch_render");
+        if (rc != VR_OK) return bfail(b, rc, "view " + std::to_string(i) + ": " + c->err);
     }
     BHIPCHK(b, hipSetDevice(b->device));
     vr_batch::Slot& sl = b->slot[b->frameNo % vr_batch::kSlots];
@@ -1384,10 +1387,15 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
         }
         sl.inFlight = false;
     }
+    // the views' own state (hit-buffer swap, frame counters) changes only
+    // once the whole frame is queued: a failed call leaves every view as it was
+    std::vector<Camera> cams(n);
     for (uint32_t i = 0, li = 0; i < n; i++) {
         vr_ctx* c = b->views[i];
-        c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
+        c->cur = 1 - c->cur;  // arguments of the swapped hit buffers (world.cpp:2221-2223)
         FrameArgs a = frame_args(c, &params[i]);
+        c->cur = 1 - c->cur;
+        cams[i] = a.cam;
         a.tileOrder = VR_TILES_LINEAR;
         sl.chunks[i / b->chunkViews].v[i % b->chunkViews] = a;
         if (c->cfg.source_pdf == VR_SOURCE_LEARNED) {
@@ -1395,11 +1403,6 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
             pa.a = a, pa.m = mlp_view(c), pa.candIdx = c->candIdx, pa.candPdf = c->candPdf;
             pa.salt = proposal_salt(params[i].R0);
         }
-        c->lastCam = a.cam;
-        c->lastSlot = -1;
-        c->initRecorded = false;  // this frame's passes run on the batch's streams
-        c->frameNo++;
-        c->framed = true;
         // uploads queued on the view's own stream (commits, lights) first
         BHIPCHK(b, hipEventRecord(b->viewReady, c->stream));
         BHIPCHK(b, hipStreamWaitEvent(b->stream, b->viewReady, 0));
@@ -1459,6 +1462,15 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
     sl.inFlight = true;
     // later work of each view (read-backs, commits) follows the frame
     for (vr_ctx* c : b->views) BHIPCHK(b, hipStreamWaitEvent(c->stream, sl.done, 0));
+    for (uint32_t i = 0; i < n; i++) {
+        vr_ctx* c = b->views[i];
+        c->cur = 1 - c->cur;  // primaryHitBuffer swap, world.cpp:2221-2223
+        c->lastCam = cams[i];
+        c->lastSlot = -1;
+        c->initRecorded = false;  // this frame's passes ran on the batch's streams
+        c->frameNo++;
+        c->framed = true;
+    }
     b->frameNo++;
     return VR_OK;
 }

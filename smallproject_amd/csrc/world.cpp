@@ -376,7 +376,7 @@ void World::Commit()
     if (!EnsureContext()) return;
     std::vector<uint32_t> ids;
     std::vector<uint32_t> payload;
-    uint32_t tasks = 0;
+    uint32_t tasks = 0, words = 0;
     for (uint32_t j = 0; j < cap / 32; j++) {
         if (!modified[j]) continue;
         for (uint32_t k = 0; k < 32; k++) {
@@ -386,11 +386,14 @@ void World::Commit()
             payload.insert(payload.end(), bricks + (size_t)i * 512u, bricks + (size_t)i * 512u + 512u);
             tasks++;
         }
-        modified[j] = 0;
+        words = j + 1;
         if (tasks + 32 >= VR_MAXCOMMITS) break;
     }
     if (tasks > 0 || firstFrame || gridDirty) {
+        // the dirty bits of the sent bricks are cleared only once the upload
+        // succeeded: after a failure they are sent again by the next Commit
         if (Check(vr_commit(ctx, grid.data(), ids.data(), payload.data(), tasks), "Commit") != VR_OK) return;
+        std::fill(modified.begin(), modified.begin() + words, 0u);
         firstFrame = false, gridDirty = false;
     }
     if (lightsDirty) {

@@ -150,3 +150,50 @@ def test_learned_source_needs_clusters(gpu):
     w = _world(dataclasses.replace(SC, source=_lib.SOURCE_LEARNED))
     _rejects(_lib.VR_ERR_STATE, w.renderer.render, w.prepare_frame(), match="vr_set_light_clusters")
     w.close()
+
+
+def test_new_light_list_drops_the_cluster_tables(gpu):
+    """vr_set_light_clusters describes one light list: after vr_set_lights
+    uploads another (of the same size here) a learned frame is refused until
+    the clusters are set again, and renders once they are."""
+    w = _world(dataclasses.replace(SC, source=_lib.SOURCE_LEARNED))
+    r = w.renderer
+    nl = len(w.lights())
+    r.set_light_clusters(np.arange(nl, dtype=np.uint32) % 4, 4)
+    p = w.prepare_frame()
+    r.render(p)
+    r.sync()
+    lights = r.read_lights(nl)
+    r.set_lights(lights[::-1].copy())
+    q = _lib.VrRenderParams.from_buffer_copy(p)
+    _rejects(_lib.VR_ERR_STATE, r.render, q, match="vr_set_light_clusters")
+    r.set_light_clusters(np.arange(nl, dtype=np.uint32) % 4, 4)
+    r.render(q)
+    r.sync()
+    w.close()
+
+
+def test_batch_keeps_the_check_codes_and_view_state(gpu):
+    """A batch frame refused for one view reports that view's own status
+    code (state, unsupported or argument) and changes no view: the views
+    then render the frames they would have rendered."""
+    wa, wb = _world(), _world()
+    b = Batch([wa, wb])
+    pa, pb = wa.prepare_frame(), wb.prepare_frame()
+    qa = _lib.VrRenderParams.from_buffer_copy(pa)
+    qa.skyDomeSampling = 1
+    _rejects(_lib.VR_ERR_UNSUPPORTED, b.render_params, [qa, pb])
+    qb = _lib.VrRenderParams.from_buffer_copy(pb)
+    qb.numberOfLights += 1
+    _rejects(_lib.VR_ERR_ARG, b.render_params, [pa, qb])
+    b.render_params([pa, pb])
+    b.sync()
+    got = [wa.renderer.hits()[0], wb.renderer.hits()[0]]
+    b.close()
+    wa.close(), wb.close()
+    wc = _world()
+    wc.renderer.render(wc.prepare_frame())
+    want = wc.renderer.hits()[0]
+    wc.close()
+    for g in got:
+        assert np.array_equal(g, want)

@@ -245,3 +245,68 @@ def test_back_to_back_frames_equal_synchronised_frames(gpu, name, flags):
     for k, (g, e) in enumerate(zip(*outs)):
         assert np.array_equal(g, e), f"output {k}: {np.mean(g != e):.6f} differ"
 
+
+
+def test_seed_zero_pixel_face_walk(gpu):
+    """A pixel whose stream starts at 0: WangHash(61) = 0, reached with
+    R0 = 61 - 171x - 1773y (cl/ris.cl:305).  xorshift32 keeps 0 at 0, so every
+    draw of that pixel is 0: all 32 candidates are light 0 with r0 = r1 = 0
+    and r2 = 0, and RandomPointOnVoxel's walk stops at face 0 (+x) because
+    cdf[0] >= r2 (cl/tools.cl:545-556).  Where the +x face has no solid angle
+    from the shading point the weight is 1/0 -> 0.  The pixel is chosen so
+    that it is, and the frame must be the oracle's bit for bit."""
+    import dataclasses
+    sc = dataclasses.replace(configs.CONFIGS["c2_area_1024"])
+    W, H = 160, 96
+    probe = configs.build_world(sc, W, H)
+    lights = probe.lights()
+    # a camera over the terrain on the -x side of light 0, looking further
+    # toward -x: the points it sees lie behind the light's +x face
+    lp = int(lights[0]["position"])
+    lx, ly, lz = lp & 1023, lp >> 20, (lp >> 10) & 1023
+    cam = ((lx - 10.0, ly + 40.0, lz - 30.0), (lx - 60.0, ly - 10.0, lz + 20.0))
+    sc.camera = lambda f: cam
+    probe.look_at(*cam)
+    p = probe.prepare_frame()
+    probe.close()
+    view_params = O.Params.from_buffer_copy(bytes(p))
+    D = O.camera_rays(view_params, W, H)
+    E = np.array(list(p.E), np.float32)
+    w0 = configs.build_world(sc, W, H)
+    view = O.View(w0.grid(), w0.bricks())
+    w0.close()
+    v, dist, _ = O.trace_batch(view, np.tile(E, (W * H, 1)), D)
+    sp = E + D * dist[:, None]
+    l0x = float(lights[0]["position"] & 1023)
+    # fl(c.x + hs) = l0x + size; the +x face faces the point iff that is < sp.x
+    cand = np.flatnonzero((v != 0) & (sp[:, 0] < l0x + float(lights[0]["size"]) - 0.5))
+    assert cand.size > 0
+    # nearest to the image centre
+    cy, cx = np.divmod(cand, W)
+    k = int(np.argmin((cx - W // 2) ** 2 + (cy - H // 2) ** 2))
+    x, y = int(cx[k]), int(cy[k])
+    r0 = (61 - 171 * x - 1773 * y) & 0xFFFFFFFF
+    assert O.wang_hash((171 * x + 1773 * y + r0) & 0xFFFFFFFF) == 0
+    # the case is live: light 0 sampled from this pixel's offset shading point
+    # with a zero stream has weight 0
+    i = y * W + x
+    _, _, side = O.trace_batch(view, E[None], D[i][None])
+    N = np.zeros(3, np.float32)
+    N[side[0]] = -1.0 if D[i][side[0]] > 0 else 1.0
+    spo = (sp[i] + np.float32(0.1) * N).astype(np.float32)
+    size = int(lights[0]["size"])
+    lp = int(lights[0]["position"])
+    centre = np.array([lp & 1023, lp >> 20, (lp >> 10) & 1023], np.float32) + np.float32(0.5 * size)
+    seed = np.zeros(1, np.uint32)
+    inv, nl, ppos = np.zeros(1, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
+    O.lib().or_random_point_on_voxel(O.ptr(centre), O.ptr(spo), size, O.ptr(seed), O.ptr(inv), O.ptr(nl), O.ptr(ppos))
+    assert inv[0] == 0.0 and nl[0] == 1.0
+
+    def before(f, world, renderer, oview, frames):
+        if f == 0:
+            world.set_next_r0(r0)
+
+    res = run_config(sc, frames=2, width=W, height=H, before_frame=before)
+    report = "\n".join(c.report() for c in res)
+    print(f"seed-0 pixel ({x}, {y}), R0 {r0:#x}\n" + report)
+    assert all(c.ok for c in res), report
