@@ -8,12 +8,15 @@ cpu_baseline leg may import this module.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIBPATH = HERE / "liboracle.so"
+# VR_ORACLE_LIB: another build of the same sources (e.g. one with a line
+# changed on purpose, to check that a pinning test notices)
+LIBPATH = Path(os.environ.get("VR_ORACLE_LIB", str(HERE / "liboracle.so")))
 
 TRACE_ALL, TRACE_OPAQUE, TRACE_THROUGH = 0, 1, 2
 QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
@@ -94,6 +97,8 @@ def lib() -> C.CDLL:
             "or_reproject": (None, [P, u32, u32, P, P]),
             "or_build_power_table": (i32, [P, P, u32, P, P, P]),
             "or_stage_albedo": (None, [P, P, P, P, i32, i32]),
+            "or_stage_initial": (None, [P, P, P, P, P, P, P, P, P, i32, i32]),
+            "or_stage_spatial": (None, [P, P, P, P, P, P, P, i32, i32]),
             "or_state_create": (P, [u32, u32]),
             "or_state_destroy": (None, [P]),
             "or_render_frame": (None, [P, P, P, P, P, P, i32]),
@@ -307,6 +312,40 @@ def trace_batch(view: View, O: np.ndarray, D: np.ndarray, steps: int = 999999, k
     s = np.zeros(n, np.uint32)
     lib().or_trace_batch(view.ref, ptr(O), ptr(D), n, steps, kind, ptr(v), ptr(d), ptr(s))
     return v, d, s
+
+
+def stage_albedo(cfg: Config, params: Params, view: View) -> np.ndarray:
+    """renderAlbedo over the whole image: the hit buffer (RAY_DTYPE)."""
+    hits = np.zeros(cfg.width * cfg.height, RAY_DTYPE)
+    lib().or_stage_albedo(C.byref(cfg), C.byref(params), view.ref, ptr(hits), 0, cfg.height)
+    return hits
+
+
+def stage_initial(cfg: Config, params: Params, view: View, lights: np.ndarray, prev_hits: np.ndarray,
+                  hits: np.ndarray, prev_res: np.ndarray, alias: Alias | None = None) -> np.ndarray:
+    """perPixelInitialSampling over the whole image on caller-given buffers.
+    `hits` is updated in place (the pixel streams); returns the reservoirs."""
+    out = np.zeros(cfg.width * cfg.height, RES_DTYPE)
+    L = np.ascontiguousarray(lights, LIGHT_DTYPE) if len(lights) else np.zeros(1, LIGHT_DTYPE)
+    for a, dt in ((prev_hits, RAY_DTYPE), (hits, RAY_DTYPE), (prev_res, RES_DTYPE)):
+        assert a.dtype == dt and a.flags["C_CONTIGUOUS"] and a.size == out.size
+    lib().or_stage_initial(C.byref(cfg), C.byref(params), view.ref, ptr(L), C.byref(alias) if alias else None,
+                           ptr(prev_hits), ptr(hits), ptr(out), ptr(prev_res), 0, cfg.height)
+    return out
+
+
+def stage_spatial(cfg: Config, params: Params, view: View, lights: np.ndarray, hits: np.ndarray,
+                  res_in: np.ndarray) -> np.ndarray:
+    """perPixelSpatialResampling over the whole image on caller-given
+    buffers.  `hits` is updated in place (the pixel streams); returns the
+    merged reservoirs."""
+    out = np.zeros(cfg.width * cfg.height, RES_DTYPE)
+    L = np.ascontiguousarray(lights, LIGHT_DTYPE) if len(lights) else np.zeros(1, LIGHT_DTYPE)
+    for a, dt in ((hits, RAY_DTYPE), (res_in, RES_DTYPE)):
+        assert a.dtype == dt and a.flags["C_CONTIGUOUS"] and a.size == out.size
+    lib().or_stage_spatial(C.byref(cfg), C.byref(params), view.ref, ptr(L), ptr(hits), ptr(out), ptr(res_in), 0,
+                           cfg.height)
+    return out
 
 
 def power_table(cfg: Config, lights: np.ndarray):

@@ -74,6 +74,20 @@ def _phat(cfg, pos, nl, sp, n, brdf, voxel, size):
     return L().or_eval_phat(C.byref(cfg), *[O.ptr(v) for v in a], voxel, size)
 
 
+def test_zero_stream_stops_the_face_walk_at_face_0():
+    """A stream of 0 stays 0 under xorshift32, so r2 = 0.  The walk's first
+    test is cdf[0] = pdf[0] < r2 (tools.cl:549-551), false for r2 = 0: the
+    pick is face 0 (+x) whatever its solid angle.  Seen from the -x side the
+    +x face has none, so pdf = 0 and the weight is 1/0 -> 0; seen from the
+    +x side it is the face with weight 1/pdf > 0."""
+    center = np.array([10.5, 10.5, 10.5])
+    inv, nl, pos, seed = _rpov(center, [0.0, 30.0, 0.0], 1, 0)
+    assert inv == 0.0 and list(nl) == [1.0, 0.0, 0.0] and seed == 0
+    assert list(pos) == [11.0, 10.0, 10.0]  # r0 = r1 = 0: the face's corner
+    inv, nl, _, _ = _rpov(center, [40.0, 30.0, 0.0], 1, 0)
+    assert inv > 1.0 and list(nl) == [1.0, 0.0, 0.0]
+
+
 def test_phat_point_light_closed_form():
     """Point mode (VOXELSAREPOINTLIGHTS=1): p-hat = |brdf * Le/pi * cos / r^2|
     with Le = ToFloatRGB * EmitStrength (phat.cl:2-41)."""
