@@ -41,9 +41,28 @@ void vrw_set_many(vrw_world* w, const uint32_t* xyz, const uint32_t* v, size_t n
 void vrw_add_brick(vrw_world* w, uint32_t bx, uint32_t by, uint32_t bz, uint32_t v);
 void vrw_remove_brick(vrw_world* w, uint32_t bx, uint32_t by, uint32_t bz);
 void vrw_optimize(vrw_world* w);
-/* Box (template/template.cpp:121-124): x1..x2-1 etc. */
+/* Box (template/template.cpp:85-88): x1..x2-1 etc. */
 void vrw_box(vrw_world* w, int x1, int y1, int z1, int x2, int y2, int z2, uint32_t c);
+/* World::Sphere (template/world.cpp:932-945): corners (u,v,w) within r of
+ * (x,y,z), in binary32 */
 void vrw_sphere(vrw_world* w, float x, float y, float z, float r, uint32_t c);
+/* World::HDisc (template/world.cpp:949-961): the disc of radius r in the
+ * plane (uint)y */
+void vrw_hdisc(vrw_world* w, float x, float y, float z, float r, uint32_t c);
+/* FillWorld / World::Fill (template/worldapi.h:7, template/world.cpp:747-757):
+ * every top cell a solid cell of colour c, every brick free */
+void vrw_fill(vrw_world* w, uint32_t c);
+/* Copy (template/worldapi.h:21-24, template/template.cpp:92-99): the box
+ * (x1,y1,z1)..(x2,y2,z2), inclusive, to (dx,dy,dz) */
+void vrw_copy(vrw_world* w, int x1, int y1, int z1, int x2, int y2, int z2, int dx, int dy, int dz);
+/* World::Print / World::PrintZ (template/world.h:219-220, world.cpp:965-1001):
+ * text in the glyph sheet's 6x10 font along +x (Print) or +z (PrintZ).  The
+ * sheet is a PNG read on first use from the path vrw_set_font names
+ * ("assets/font.png" by default, as in World::World, world.cpp:192).  Print
+ * returns VR_ERR_ARG (and draws nothing) when the sheet cannot be read. */
+int vrw_set_font(vrw_world* w, const char* png_path);
+int vrw_print(vrw_world* w, const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c);
+int vrw_printz(vrw_world* w, const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c);
 
 /* lights (template/world.cpp:228-612) */
 uint32_t vrw_find_lights(const vrw_world* w, vr_light* out, uint32_t cap);

@@ -1,11 +1,14 @@
 // world.h - host-side brick-grid World, the reference's WrldTmpl8 API
-// (template/world.h:164-758) re-implemented over the vr_* C ABI.
+// (template/world.h:164-758) re-implemented over the vr_* C ABI
+// (vr_api.h), for C++ callers such as the reference's scene code.
 //
 // The public surface keeps the reference's names and meaning: Set/Get/
 // AddBrick/RemoveBrick/OptimizeBricks/FindLightsInWord/SetupLightBuffer/
-// AddLight/RemoveLight, the camera matrix accessors, GetRenderParams(),
-// Render() and Commit().  The host keeps the authoritative grid, brick pool,
-// zero counts, dirty bits and light list; Commit() mirrors them to the device.
+// AddLight/RemoveLight, Sphere/HDisc/Print/PrintZ/Fill, the camera matrix
+// accessors, GetRenderParams(), Render() and Commit().  The host keeps the
+// authoritative grid, brick pool, zero counts, dirty bits and light list;
+// Commit() mirrors them to the device.  worldapi.h adds the reference's
+// free functions (Plot, Read, Box, Copy, ...) over a current World.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -92,7 +95,24 @@ public:
     void Set(uint32_t x, uint32_t y, uint32_t z, uint32_t v);
     void AddBrick(uint32_t bx, uint32_t by, uint32_t bz, uint32_t v);
     void RemoveBrick(uint32_t bx, uint32_t by, uint32_t bz);
+    // World::Fill (template/world.cpp:747-757): every top cell becomes a
+    // solid cell of colour c, every brick is free again, no dirty marks
+    void Fill(uint32_t c);
+
+    // high-level voxel access (template/world.h:217-220, world.cpp:932-1000)
     void Sphere(float x, float y, float z, float r, uint32_t c);
+    void HDisc(float x, float y, float z, float r, uint32_t c);
+    // text in the glyph sheet's 6x10 font: Print along +x in the plane z,
+    // PrintZ along +z in the plane x; glyph pixels that are black in the
+    // sheet are set to c.  The sheet is read on first use from the path
+    // SetFont names (the reference loads "assets/font.png" in World::World,
+    // template/world.cpp:192); without one, Print sets LastError and draws
+    // nothing.
+    void Print(const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c);
+    void PrintZ(const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c);
+    bool SetFont(const char* pngPath);
+    // the glyph sheet is (or could now be) read: false if it cannot be
+    bool SetFontIfMissing() { return EnsureFont(); }
 
     // lights (template/world.h:245-262)
     void FindLightsInWord(std::vector<Light>& ls) const;
@@ -137,6 +157,8 @@ private:
     void FreeBrick(uint32_t idx);
     void Mark(uint32_t idx) { modified[idx >> 5] |= 1u << (idx & 31); }
     uint32_t SplitSolidBrick(uint32_t color, uint32_t cellIndex);
+    bool EnsureFont();
+    void PrintGlyphs(const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c, bool alongZ);
     int Check(int rc, const char* what);
     bool EnsureContext();
 
@@ -160,6 +182,14 @@ private:
     float stageMs[VR_NUM_STAGES] = {};
     int status = VR_OK;
     std::string error;
+    std::string fontPath = "assets/font.png";
+    std::vector<uint32_t> font;  // glyph sheet, 0xRRGGBB per pixel
+    uint32_t fontW = 0, fontH = 0;
 };
+
+// PNG reader for the glyph sheet (8-bit grey, RGB or RGBA, not interlaced):
+// pixels as 0xRRGGBB, row-major from the top, as the reference's
+// Surface::LoadImage stores them (template/template.cpp:1776-1798).
+bool ReadPngRgb(const char* path, uint32_t& w, uint32_t& h, std::vector<uint32_t>& px, std::string& err);
 
 }  // namespace vrhost

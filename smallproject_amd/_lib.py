@@ -117,6 +117,12 @@ SIGNATURES = {
     "vrw_optimize": (None, [_P]),
     "vrw_box": (None, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _u32]),
     "vrw_sphere": (None, [_P, _f32, _f32, _f32, _f32, _u32]),
+    "vrw_hdisc": (None, [_P, _f32, _f32, _f32, _f32, _u32]),
+    "vrw_fill": (None, [_P, _u32]),
+    "vrw_copy": (None, [_P] + [C.c_int] * 9),
+    "vrw_set_font": (C.c_int, [_P, C.c_char_p]),
+    "vrw_print": (C.c_int, [_P, C.c_char_p, _u32, _u32, _u32, _u32]),
+    "vrw_printz": (C.c_int, [_P, C.c_char_p, _u32, _u32, _u32, _u32]),
     "vrw_find_lights": (_u32, [_P, _P, _u32]),
     "vrw_num_lights": (_u32, [_P]),
     "vrw_get_lights": (_u32, [_P, _P, _u32]),

@@ -286,6 +286,27 @@ class World:
     def box(self, x1, y1, z1, x2, y2, z2, c):
         lib().vrw_box(self.h, x1, y1, z1, x2, y2, z2, c)
 
+    def sphere(self, x, y, z, r, c):
+        lib().vrw_sphere(self.h, x, y, z, r, c)
+
+    def hdisc(self, x, y, z, r, c):
+        lib().vrw_hdisc(self.h, x, y, z, r, c)
+
+    def fill(self, c):
+        lib().vrw_fill(self.h, c)
+
+    def copy(self, x1, y1, z1, x2, y2, z2, dx, dy, dz):
+        lib().vrw_copy(self.h, x1, y1, z1, x2, y2, z2, dx, dy, dz)
+
+    def set_font(self, path):
+        self._chk(lib().vrw_set_font(self.h, str(path).encode()), "set_font")
+
+    def print(self, text: str, x, y, z, c):
+        self._chk(lib().vrw_print(self.h, text.encode("latin-1"), x, y, z, c), "print")
+
+    def printz(self, text: str, x, y, z, c):
+        self._chk(lib().vrw_printz(self.h, text.encode("latin-1"), x, y, z, c), "printz")
+
     # lights
     def lights(self) -> np.ndarray:
         n = lib().vrw_num_lights(self.h)

@@ -17,6 +17,8 @@ and render settings.  `build_world` constructs the host World (CPU only);
   c5_dynamic_1024  configuration 2's world with 16,384 emitters moving at up
                  to 3 voxels/frame and 5% of the bricks edited per frame on
                  the device, MIS-weighted spatial merge, 120 frames
+  c1_translucent_128  configuration 1's scene with 400 translucent panes
+                 (alpha 3..12 of 15) over the terrain
   batch_instances()  16 scene instances of uneven world size (256^3 -
                  1024^3) and resolution (320x180 - 1920x1080) rendered as one
                  batch of views
@@ -69,6 +71,9 @@ class SceneConfig:
     # scene motion on the device (vr_scene_step): light speeds uniform in
     # [0, light_speed] voxels/frame in uniform directions (motion_seed);
     # edit_permille of the bricks edited per frame by walk_steps-step walks
+    # translucent panes (alpha 3..12 of 15) over the terrain, pane_seed
+    panes: int = 0
+    pane_seed: int = 0
     light_speed: float = 0.0
     motion_seed: int = 0
     edit_permille: int = 0
@@ -106,6 +111,12 @@ CONFIGS = {
         light_mode="surface", emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=6,
         flags=_lib.FLAG_MIS_SPATIAL, light_speed=3.0, motion_seed=9, edit_permille=50, walk_steps=32,
         edit_seed=10),
+    # configuration 1 with 400 translucent panes: renderAlbedo's
+    # compositing loop and shadow rays through translucent voxels
+    "c1_translucent_128": SceneConfig(
+        name="c1_translucent_128", width=640, height=360, point=True, fov=60.0, frames=8, world_size=128, scale=1,
+        terrain_seed=1, lights=64, light_seed=2, light_mode="above", light_height=(2, 20), emit_range=(1, 10),
+        camera_kind=0, camera_seed=3, panes=400, pane_seed=13),
 }
 
 
@@ -113,13 +124,16 @@ BATCH_RESOLUTIONS = ((320, 180), (640, 360), (1280, 720), (1920, 1080))
 BATCH_WORLDS = (256, 512, 1024)
 
 
-def batch_instances(n: int = 16, seed: int = 8, frames: int = 30) -> list[SceneConfig]:
+def batch_instances(n: int = 16, seed: int = 8, frames: int = 30, learned_every: int = 0) -> list[SceneConfig]:
     """The batched-views configuration: n independent scene instances, each
     a seeded heightfield + boxes world of edge 256, 512 or 1024 (heightfield
     scale edge/128, 2,000 boxes per 1024^2 of area), a resolution from
     BATCH_RESOLUTIONS, 256-16,384 emissive surface voxels (log-uniform count,
     intensities log-uniform in [0.1, 10]), area lights and its own spline
-    camera path.  Drawn with numpy's PCG64 generator seeded with `seed`."""
+    camera path.  Drawn with numpy's PCG64 generator seeded with `seed`.
+    learned_every = k > 0: instance i with i % k == k - 1 samples its
+    candidates from its own scene's learned proposal (clusters: its lights
+    in min(256, max(8, lights/64)) k-means clusters)."""
     import numpy as np
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
@@ -134,6 +148,9 @@ def batch_instances(n: int = 16, seed: int = 8, frames: int = 30) -> list[SceneC
             box_edge=(4, 32), box_seed=base + 1, lights=lights, light_seed=base + 2, light_mode="surface",
             emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=base + 3,
             capacity={256: 1 << 15, 512: 1 << 17, 1024: 1 << 19}[size]))
+        if learned_every and i % learned_every == learned_every - 1:
+            out[-1].source = _lib.SOURCE_LEARNED
+            out[-1].clusters = min(256, max(8, lights // 64))
     return out
 
 
@@ -178,6 +195,35 @@ def scene_step(sc: SceneConfig, world: World, frame: int):
     world.renderer.scene_step(frame, sc.edit_seed, sc.edit_permille, sc.walk_steps, world.high_water())
 
 
+def add_panes(world: World, sc: SceneConfig):
+    """sc.panes translucent panes, one voxel thick: horizontal ones 4-16
+    voxels above the terrain and vertical ones standing on it, each 6-24
+    voxels on a side, of one colour with alpha 3..12 of 15 (so primary rays
+    composite through them and shadow rays pass them); numpy PCG64 seeded
+    with sc.pane_seed."""
+    import numpy as np
+    from .api import terrain_height
+    rng = np.random.Generator(np.random.PCG64(sc.pane_seed))
+    size = sc.world_size
+    for _ in range(sc.panes):
+        a, b = (int(v) for v in rng.integers(6, 25, 2))
+        x0, z0 = (int(v) for v in rng.integers(0, size - 24, 2))
+        h = int(terrain_height(x0 + a / 2, z0 + b / 2, sc.scale, sc.terrain_seed))
+        word = (int(rng.integers(3, 13)) << 12) | int(rng.integers(1, 0x1000))
+        if rng.random() < 0.6:
+            y = min(h + int(rng.integers(4, 17)), 1023)
+            xs, zs = np.meshgrid(np.arange(x0, x0 + a), np.arange(z0, z0 + b), indexing="ij")
+            xyz = np.stack([xs.ravel(), np.full(xs.size, y), zs.ravel()], axis=1)
+        elif rng.random() < 0.5:
+            xs, ys = np.meshgrid(np.arange(x0, x0 + a), np.arange(h - 2, h - 2 + b), indexing="ij")
+            xyz = np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, z0)], axis=1)
+        else:
+            zs, ys = np.meshgrid(np.arange(z0, z0 + a), np.arange(h - 2, h - 2 + b), indexing="ij")
+            xyz = np.stack([np.full(zs.size, x0), ys.ravel(), zs.ravel()], axis=1)
+        xyz = xyz[(xyz[:, 1] >= 0) & (xyz[:, 1] < 1024)]
+        world.set_many(xyz, np.full(len(xyz), word, np.uint32))
+
+
 def build_world(sc: SceneConfig | str, width: int | None = None, height: int | None = None) -> World:
     """Build the host World of a configuration (no GPU needed)."""
     if isinstance(sc, str):
@@ -190,6 +236,8 @@ def build_world(sc: SceneConfig | str, width: int | None = None, height: int | N
         if sc.boxes:
             w.scene_boxes(sc.boxes, sc.box_edge[0], sc.box_edge[1], sc.world_size, sc.scale, sc.terrain_seed,
                           sc.box_seed)
+    if sc.panes:
+        add_panes(w, sc)
     w.optimize()
     if sc.lights:
         if sc.light_mode == "clusters":

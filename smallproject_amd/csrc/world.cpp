@@ -7,6 +7,8 @@
 
 #include <sys/mman.h>
 
+#include "worldapi.h"
+
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -240,19 +242,114 @@ void World::RemoveBrick(uint32_t bx, uint32_t by, uint32_t bz)
     Mark(0);
 }
 
+// the corner range [lo, hi) of a primitive along one axis: (uint)max(0, p - r)
+// to (uint)min(dim - 1, p + r + 1), as World::Sphere and World::HDisc take it
+// (template/world.cpp:934-936, 951-952)
+static inline void span(float p, float r, uint32_t& lo, uint32_t& hi)
+{
+    lo = (uint32_t)std::fmax(0.0f, p - r);
+    hi = (uint32_t)std::fmin((float)VR_MAPDIM - 1.0f, p + r + 1.0f);
+}
+
 void World::Sphere(float x, float y, float z, float r, uint32_t c)
 {
-    // World::Sphere semantics: every voxel whose centre lies inside the ball
-    const int x0 = (int)std::floor(x - r), x1 = (int)std::ceil(x + r);
-    const int y0 = (int)std::floor(y - r), y1 = (int)std::ceil(y + r);
-    const int z0 = (int)std::floor(z - r), z1 = (int)std::ceil(z + r);
-    for (int yy = y0; yy <= y1; yy++)
-        for (int zz = z0; zz <= z1; zz++)
-            for (int xx = x0; xx <= x1; xx++) {
-                if (xx < 0 || yy < 0 || zz < 0) continue;
-                const float dx = (float)xx + 0.5f - x, dy = (float)yy + 0.5f - y, dz = (float)zz + 0.5f - z;
-                if (dx * dx + dy * dy + dz * dz < r * r) Set((uint32_t)xx, (uint32_t)yy, (uint32_t)zz, c);
+    // World::Sphere, template/world.cpp:932-945: voxel corners (u, v, w)
+    // with |(u, v, w) - (x, y, z)|^2 < r^2, in binary32
+    uint32_t x1, x2, y1, y2, z1, z2;
+    span(x, r, x1, x2), span(y, r, y1, y2), span(z, r, z1, z2);
+    if (x1 >= VR_MAPDIM || y1 >= VR_MAPDIM || z1 >= VR_MAPDIM || x2 >= VR_MAPDIM || y2 >= VR_MAPDIM ||
+        z2 >= VR_MAPDIM)
+        return;
+    const float r2 = r * r;
+    for (uint32_t u = x1; u < x2; u++)
+        for (uint32_t v = y1; v < y2; v++)
+            for (uint32_t w = z1; w < z2; w++) {
+                const float dx = (float)u - x, dy = (float)v - y, dz = (float)w - z;
+                if (dx * dx + dy * dy + dz * dz < r2) Set(u, v, w, c);
             }
+}
+
+void World::HDisc(float x, float y, float z, float r, uint32_t c)
+{
+    // World::HDisc, template/world.cpp:949-961: the horizontal disc in the
+    // plane (uint)y
+    uint32_t x1, x2, z1, z2;
+    span(x, r, x1, x2), span(z, r, z1, z2);
+    if (x1 >= VR_MAPDIM || !(y < (float)VR_MAPDIM) || z1 >= VR_MAPDIM || x2 >= VR_MAPDIM || z2 >= VR_MAPDIM) return;
+    const float r2 = r * r;
+    for (uint32_t u = x1; u < x2; u++)
+        for (uint32_t w = z1; w < z2; w++) {
+            const float dx = (float)u - x, dz = (float)w - z;
+            if (dx * dx + dz * dz < r2) Set(u, (uint32_t)y, w, c);
+        }
+}
+
+void World::Fill(uint32_t c)
+{
+    // World::Fill, template/world.cpp:747-757 (the lights are untouched)
+    std::fill(grid.begin(), grid.end(), c << 1);
+    for (uint32_t i = 0; i < cap; i++) trash[(i * 31u) & (cap - 1)] = i;
+    trashHead = cap, trashTail = 0;
+    std::fill(modified.begin(), modified.end(), 0u);
+    gridDirty = true;
+}
+
+bool World::SetFont(const char* pngPath)
+{
+    if (!pngPath) return false;
+    fontPath = pngPath;
+    font.clear();
+    return EnsureFont();
+}
+
+bool World::EnsureFont()
+{
+    if (!font.empty()) return true;
+    std::string err;
+    if (!ReadPngRgb(fontPath.c_str(), fontW, fontH, font, err)) {
+        font.clear();
+        status = VR_ERR_ARG;
+        error = "font: " + err;
+        return false;
+    }
+    return true;
+}
+
+void World::PrintGlyphs(const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c, bool alongZ)
+{
+    // World::Print / World::PrintZ, template/world.cpp:965-1001: glyph t of
+    // the table below is the 6x10 block at column t % 13, row t / 13 of the
+    // sheet; a character not in the table draws glyph 0, one at or below
+    // ' ' (or above 127) draws nothing but still takes its 6 voxels.
+    static const char table[] =
+        "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz1234567890+-=/\\*:;()[]{}<>!?.,'\"&#$_%^|";
+    if (!text || !EnsureFont()) return;
+    int glyph[256] = {};
+    for (int i = 0; table[i]; i++) glyph[(unsigned char)table[i]] = i;
+    const size_t n = std::strlen(text);
+    for (size_t i = 0; i < n; i++) {
+        const signed char ch = (signed char)text[i];
+        if (ch <= 32) continue;
+        const int t = glyph[(unsigned char)ch], cx = t % 13, cy = t / 13;
+        for (uint32_t v = 0; v < 10; v++)
+            for (uint32_t u = 0; u < 6; u++) {
+                const uint32_t sx = (uint32_t)cx * 6 + u, sy = (uint32_t)cy * 10 + v;
+                if (sx >= fontW || sy >= fontH || (font[sx + sy * fontW] & 0xffffffu) != 0u) continue;
+                const uint32_t along = u + (uint32_t)i * 6;
+                if (alongZ) Set(x, y + (9 - v), z + along, c);
+                else Set(x + along, y + (9 - v), z, c);
+            }
+    }
+}
+
+void World::Print(const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c)
+{
+    PrintGlyphs(text, x, y, z, c, false);
+}
+
+void World::PrintZ(const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c)
+{
+    PrintGlyphs(text, x, y, z, c, true);
 }
 
 void World::OptimizeBricks()
@@ -480,6 +577,32 @@ void World::Render()
     if (!BeginFrame()) return;
     if (Check(vr_render_frame(ctx, &params), "Render") != VR_OK) return;
     FinishFrame();
+}
+
+// ---------------------------------------------------------------------------
+// the free-function world API (worldapi.h) over the current World
+// ---------------------------------------------------------------------------
+static World* g_world = nullptr;
+
+World* GetWorld() { return g_world; }
+void SetWorld(World* w) { g_world = w; }
+
+void Box(const int x1, const int y1, const int z1, const int x2, const int y2, const int z2, const uint32_t c)
+{
+    // Box, template/template.cpp:85-88
+    for (int y = y1; y < y2; y++)
+        for (int z = z1; z < z2; z++)
+            for (int x = x1; x < x2; x++) Plot((uint32_t)x, (uint32_t)y, (uint32_t)z, c);
+}
+
+void Copy(const int3 s1, const int3 s2, const int3 D)
+{
+    // Copy, template/template.cpp:92-99
+    const int3 e = int3{s2.x - s1.x, s2.y - s1.y, s2.z - s1.z};
+    for (int z = 0; z <= e.z; z++)
+        for (int y = 0; y <= e.y; y++)
+            for (int x = 0; x <= e.x; x++)
+                Plot(int3{D.x + x, D.y + y, D.z + z}, Read(int3{s1.x + x, s1.y + y, s1.z + z}));
 }
 
 }  // namespace vrhost

@@ -4,6 +4,7 @@
 
 #include "vr_world.h"
 #include "world.h"
+#include "worldapi.h"
 
 using vrhost::World;
 
@@ -55,6 +56,36 @@ void vrw_box(vrw_world* w, int x1, int y1, int z1, int x2, int y2, int z2, uint3
             for (int x = x1; x < x2; x++) W(w)->Set((uint32_t)x, (uint32_t)y, (uint32_t)z, c);
 }
 void vrw_sphere(vrw_world* w, float x, float y, float z, float r, uint32_t c) { W(w)->Sphere(x, y, z, r, c); }
+void vrw_hdisc(vrw_world* w, float x, float y, float z, float r, uint32_t c) { W(w)->HDisc(x, y, z, r, c); }
+void vrw_fill(vrw_world* w, uint32_t c) { W(w)->Fill(c); }
+void vrw_copy(vrw_world* w, int x1, int y1, int z1, int x2, int y2, int z2, int dx, int dy, int dz)
+{
+    vrhost::World* prev = vrhost::GetWorld();
+    vrhost::SetWorld(W(w));
+    vrhost::Copy(vrhost::int3{x1, y1, z1}, vrhost::int3{x2, y2, z2}, vrhost::int3{dx, dy, dz});
+    vrhost::SetWorld(prev);
+}
+int vrw_set_font(vrw_world* w, const char* png_path)
+{
+    if (!w || !png_path) return VR_ERR_ARG;
+    return W(w)->SetFont(png_path) ? VR_OK : VR_ERR_ARG;
+}
+static int print_text(vrw_world* w, const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c, bool alongZ)
+{
+    if (!w || !text) return VR_ERR_ARG;
+    if (!W(w)->SetFontIfMissing()) return VR_ERR_ARG;
+    if (alongZ) W(w)->PrintZ(text, x, y, z, c);
+    else W(w)->Print(text, x, y, z, c);
+    return VR_OK;
+}
+int vrw_print(vrw_world* w, const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c)
+{
+    return print_text(w, text, x, y, z, c, false);
+}
+int vrw_printz(vrw_world* w, const char* text, uint32_t x, uint32_t y, uint32_t z, uint32_t c)
+{
+    return print_text(w, text, x, y, z, c, true);
+}
 
 uint32_t vrw_find_lights(const vrw_world* w, vr_light* out, uint32_t cap)
 {

@@ -74,6 +74,51 @@ def params_for_oracle(p) -> O.Params:
     return O.Params.from_buffer_copy(bytes(p))
 
 
+class OracleMirror:
+    """The oracle's copy of one World's frames: same world, lights, emission
+    LUT and configuration, rendered with the params the GPU frame used."""
+
+    def __init__(self, w, sc: configs.SceneConfig):
+        W, H = w.cfg.width, w.cfg.height
+        self.learned = sc.source == _lib.SOURCE_LEARNED
+        self.view = O.View(w.grid(), w.bricks())
+        lut = np.array(list(w.cfg.emit_lut), np.float32)
+        ocfg = O.make_config(W, H, sc.point, gfxexp=1, quirks=sc.quirks,
+                             source=O.SOURCE_SLOTS if self.learned else sc.source, lut=lut,
+                             spatial_mis=bool(sc.flags & _lib.FLAG_MIS_SPATIAL))
+        self.frames = O.Frames(ocfg, self.view, w.lights())
+        self.nlights = len(w.lights())
+
+    def render(self, p, r):
+        """The frame with params p; r (the GPU renderer that rendered it)
+        supplies the learned source's candidate slots."""
+        if self.learned:
+            self.frames.set_slots(*r.candidates(), r.cluster_tables(self.nlights))
+        self.frames.render(params_for_oracle(p))
+
+    def compare(self, tag: str, r, strict: bool = True) -> "Comparison":
+        of = self.frames
+        cmp = Comparison()
+        hw, d, s = r.hits()
+        oh = of.hits()
+        cmp.exact(f"{tag} hitword", hw, expected_hitword(oh))
+        cmp.exact(f"{tag} dist", d, oh["dist"])
+        cmp.exact(f"{tag} seed", s, oh["seed"])
+        for which, ores in ((1, of.initial()), (0, of.final())):
+            a, b, m = r.reservoirs(which)
+            ea, eb, em = split_reservoirs(ores)
+            kind = "initial" if which == 1 else "final"
+            cmp.exact(f"{tag} {kind} M", m & 0xFFFFFF, em & 0xFFFFFF)
+            cmp.exact(f"{tag} {kind} light", b[:, 3].view(np.uint32), eb[:, 3].view(np.uint32))
+            cmp.exact(f"{tag} {kind} face", m >> 24, em >> 24)
+            for k, nm in enumerate(("wsum", "W", "pHat", "invPdf")):
+                (cmp.exact if strict else cmp.tol)(f"{tag} {kind} {nm}", a[:, k], ea[:, k])
+            (cmp.exact if strict else cmp.tol)(f"{tag} {kind} pos", b[:, :3], eb[:, :3])
+        fr = r.frame()
+        (cmp.exact if strict else cmp.tol)(f"{tag} radiance", fr[:, :3], of.frame()[:, :3])
+        return cmp
+
+
 def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, height: int | None = None,
                quirks: int | None = None, source: int | None = None, check_every: bool = True, setup=None,
                flags: int | None = None, before_frame=None, strict: bool = True):
@@ -94,49 +139,22 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
                                  source=sc.source if source is None else source,
                                  flags=sc.flags if flags is None else flags)
     w = configs.build_world(sc, width, height)
-    W, H = w.cfg.width, w.cfg.height
     w.force_sync()
     r = w.renderer
     if setup is not None:
         setup(w, r)
-    learned = sc.source == _lib.SOURCE_LEARNED
-    view = O.View(w.grid(), w.bricks())
-    lut = np.array(list(w.cfg.emit_lut), np.float32)
-    ocfg = O.make_config(W, H, sc.point, gfxexp=1, quirks=sc.quirks,
-                         source=O.SOURCE_SLOTS if learned else sc.source, lut=lut,
-                         spatial_mis=bool(sc.flags & _lib.FLAG_MIS_SPATIAL))
-    of = O.Frames(ocfg, view, w.lights())
+    mirror = OracleMirror(w, sc)
     results = []
     for f in range(frames):
         if before_frame is not None:
-            before_frame(f, w, r, view, of)
+            before_frame(f, w, r, mirror.view, mirror.frames)
         pos, tgt = sc.camera(f)
         w.look_at(pos, tgt)
         p = w.prepare_frame()
         w.finish_frame()
         r.render(p)
-        if learned:
-            of.set_slots(*r.candidates(), r.cluster_tables(len(w.lights())))
-        of.render(params_for_oracle(p))
+        mirror.render(p, r)
         if not (check_every or f == frames - 1):
             continue
-        cmp = Comparison()
-        hw, d, s = r.hits()
-        oh = of.hits()
-        cmp.exact(f"f{f} hitword", hw, expected_hitword(oh))
-        cmp.exact(f"f{f} dist", d, oh["dist"])
-        cmp.exact(f"f{f} seed", s, oh["seed"])
-        for which, ores in ((1, of.initial()), (0, of.final())):
-            a, b, m = r.reservoirs(which)
-            ea, eb, em = split_reservoirs(ores)
-            tag = "initial" if which == 1 else "final"
-            cmp.exact(f"f{f} {tag} M", m & 0xFFFFFF, em & 0xFFFFFF)
-            cmp.exact(f"f{f} {tag} light", b[:, 3].view(np.uint32), eb[:, 3].view(np.uint32))
-            cmp.exact(f"f{f} {tag} face", m >> 24, em >> 24)
-            for k, nm in enumerate(("wsum", "W", "pHat", "invPdf")):
-                (cmp.exact if strict else cmp.tol)(f"f{f} {tag} {nm}", a[:, k], ea[:, k])
-            (cmp.exact if strict else cmp.tol)(f"f{f} {tag} pos", b[:, :3], eb[:, :3])
-        fr = r.frame()
-        (cmp.exact if strict else cmp.tol)(f"f{f} radiance", fr[:, :3], of.frame()[:, :3])
-        results.append(cmp)
+        results.append(mirror.compare(f"f{f}", r, strict))
     return results

@@ -34,7 +34,7 @@ def _rays(rng, n, lo, hi, target=None):
     return O_, D.astype(np.float32)
 
 
-@pytest.mark.parametrize("name", ["c1_point_128", "c2_area_1024"])
+@pytest.mark.parametrize("name", ["c1_point_128", "c2_area_1024", "c1_translucent_128"])
 def test_dda_matches_oracle(gpu, name):
     """TraceRay / TraceOpaqueRay (cl/trace.cl:205-362) on random rays, from
     inside the world and from outside the map (slab clip), with the primary
@@ -72,6 +72,19 @@ def test_c2_area_lights_reduced_resolution(gpu):
     """Configuration 2's world and lights (1024^3, 4096 area emitters,
     log intensities) at 320x180 for 3 frames."""
     res = run_config(configs.CONFIGS["c2_area_1024"], frames=3, width=320, height=180)
+    report = "\n".join(c.report() for c in res)
+    print(report)
+    assert all(c.ok for c in res), report
+
+
+@pytest.mark.parametrize("flags", [0, _lib.FLAG_MIS_SPATIAL], ids=["biased", "mis"])
+def test_translucent_panes_match_oracle(gpu, flags):
+    """Configuration 1 with 400 translucent panes: primary rays composite
+    through up to 25 surfaces (renderAlbedo's TraceThrough loop,
+    cl/ris.cl:318-389) and every shadow ray passes translucent voxels
+    (TraceOpaqueRay, cl/trace.cl:116).  Hit word (composited colour and
+    visibility), face, distance, reservoirs and radiance bit-exact."""
+    res = run_config(configs.CONFIGS["c1_translucent_128"], frames=4, width=320, height=180, flags=flags)
     report = "\n".join(c.report() for c in res)
     print(report)
     assert all(c.ok for c in res), report
