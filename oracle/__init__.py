@@ -398,7 +398,11 @@ class Frames:
         self.cfg.cl_start, self.cfg.cl_lights, self.cfg.cl_prob = ptr(t["start"]), ptr(t["lights"]), ptr(t["prob"])
         self.cfg.cl_alias, self.cfg.cl_share = ptr(t["alias"]), ptr(t["share"])
 
-    def render(self, params: Params, threads: int = 8):
+    def render(self, params: Params, threads: int | None = None):
+        """One frame.  threads: OpenMP threads over pixel rows (default:
+        VR_ORACLE_THREADS, else 8)."""
+        if threads is None:
+            threads = int(os.environ.get("VR_ORACLE_THREADS", "8"))
         assert self.cfg.source_pdf != SOURCE_SLOTS or self.cfg.cand_idx, "SOURCE_SLOTS needs set_slots()"
         lib().or_render_frame(self.h, C.byref(self.cfg), C.byref(params), self.view.ref, ptr(self.lights),
                               C.byref(self.alias) if self.alias is not None else None, threads)

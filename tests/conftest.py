@@ -7,6 +7,13 @@ import pytest
 ROOT = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(ROOT))
 
+# Under pytest-xdist each worker process runs the oracle's OpenMP loops:
+# one thread each, and no spinning, so the workers do not oversubscribe the
+# cores (set before the oracle library starts its thread pool).
+if "PYTEST_XDIST_WORKER" in os.environ:
+    os.environ.setdefault("VR_ORACLE_THREADS", "1")
+    os.environ.setdefault("OMP_WAIT_POLICY", "passive")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")

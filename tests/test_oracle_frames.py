@@ -32,7 +32,7 @@ def _render(w, frames, quirks=O.QUIRKS_REFERENCE, point=True):
         w.look_at([40.0, 60.0, 30.0], [64.0, 8.0, 70.0])
         p = w.prepare_frame()
         w.finish_frame()
-        fr.render(O.Params.from_buffer_copy(bytes(p)), threads=4)
+        fr.render(O.Params.from_buffer_copy(bytes(p)))
     return fr, p
 
 
@@ -121,7 +121,7 @@ def _mean_images(point, spatial, mis, K, salt):
         p = O.Params.from_buffer_copy(bytes(w.prepare_frame()))
         w.finish_frame()
         p.temporal, p.spatial = 0, int(spatial)
-        fr.render(p, threads=8)
+        fr.render(p)
         imgs[f] = fr.frame()[:, :3]
     return imgs
 
