@@ -262,6 +262,19 @@ int vr_profile_read(vr_ctx* ctx, double sum_ms[VR_NUM_STAGES], uint32_t* frames)
 int vr_trace_rays(vr_ctx* ctx, const float* origins, const float* dirs, uint32_t n,
                   uint32_t kind, int32_t steps, uint32_t* voxel, float* dist, uint32_t* side);
 
+/* vr_trace_rays without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The rays and hits are staged in a fixed device scratch of
+ * VR_NOALLOC_MAX_RAYS rays (9 MiB per device, held until the process ends);
+ * larger n is rejected with VR_ERR_ARG.  Same arguments, checks and results
+ * as vr_trace_rays.  Every context on a device shares that device's
+ * scratch: calls that overlap on one device (from several threads, or on
+ * several contexts' streams) race and must not be made. */
+#define VR_NOALLOC_MAX_RAYS (1u << 18)
+int vr_trace_rays_noalloc(vr_ctx* ctx, const float* origins, const float* dirs, uint32_t n,
+                          uint32_t kind, int32_t steps, uint32_t* voxel, float* dist, uint32_t* side);
+
 /* ---- learned proposal (VR_SOURCE_LEARNED; no counterpart in the reference) ----
  * A 9-64-64-256 MLP maps a shading point (position / 1024, one-hot voxel
  * face) to a softmax q over light clusters.  A candidate is a cluster drawn

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -29,10 +30,45 @@ static constexpr int kRing = 64;  // frames whose stage events are kept
 // view, 6.7 -> 8.3 ms per round).
 static std::atomic<int> g_liveContexts{0};
 
+// Scratch of vr_trace_rays_noalloc: VR_NOALLOC_MAX_RAYS rays and their hits
+// staged on the device (origins and directions 12 B each, voxel, distance and
+// side 4 B each: 36 B per ray, 9 MiB).  Every device the library runs on
+// holds its own copy from module load until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (scratch_address), never through the array's name.
+static __device__ __align__(256) unsigned char scratch[(size_t)VR_NOALLOC_MAX_RAYS * 36];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMu;
+static void* g_scratchPtr[kMaxDevices] = {};
+
+// the device address of `scratch` on `device`, which must be current; looked
+// up once per device.  nullptr (and *err set) when the lookup fails.
+static void* scratch_address(int device, hipError_t* err)
+{
+    *err = hipSuccess;
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        g_scratchPtr[device] = p;
+    }
+    return g_scratchPtr[device];
+}
+
 struct vr_ctx {
     vr_config cfg;
     int device = 0;
     hipStream_t stream = nullptr;
+    // this device's `scratch` (vr_trace_rays_noalloc), set by vr_create;
+    // nullptr if its lookup failed (noScratch holds why)
+    void* scratch = nullptr;
+    hipError_t noScratch = hipSuccess;
     // The next frame's primary pass may run on `side` beside this frame's
     // spatial reuse + shading: it writes the hit buffer that only this
     // frame's candidate pass read (`initDone`), unless work queued since then
@@ -485,6 +521,9 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
         vr_destroy(c);
         return VR_ERR_NOMEM;
     }
+    // set-up of the allocation-free entry point, with the context's device
+    // current: a failed lookup leaves every other entry point working
+    c->scratch = scratch_address(c->device, &c->noScratch);
     g_liveContexts++;
     c->counted = true;
     *out = c;
@@ -802,6 +841,48 @@ int vr_trace_rays(vr_ctx* c, const float* O, const float* D, uint32_t n, uint32_
     (void)hipStreamSynchronize(c->stream);
     dfree(dO), dfree(dD), dfree(dd), dfree(dv), dfree(ds);
     return rc;
+}
+
+int vr_trace_rays_noalloc(vr_ctx* c, const float* O, const float* D, uint32_t n, uint32_t kind, int32_t steps,
+                          uint32_t* voxel, float* dist, uint32_t* side)
+{
+    // vr_trace_rays with the rays and hits staged in the device's `scratch`
+    // instead of buffers allocated per call: no hipMalloc or hipFree, so the
+    // call suits callers that cannot allocate.  The kernel is the same one.
+    if (!c) return VR_ERR_ARG;
+    if (n == 0) return VR_OK;
+    if (!O || !D || !voxel || !dist || !side || kind > VR_TRACE_OPAQUE)
+        return fail(c, VR_ERR_ARG, "vr_trace_rays_noalloc: bad arguments");
+    if (n > VR_NOALLOC_MAX_RAYS)
+        return fail(c, VR_ERR_ARG, "vr_trace_rays_noalloc: " + std::to_string(n) + " rays exceed the scratch's " +
+                                       std::to_string(VR_NOALLOC_MAX_RAYS));
+    if (!c->committed) return fail(c, VR_ERR_STATE, "vr_trace_rays_noalloc: no world committed");
+    if (!c->scratch) return hipfail(c, c->noScratch, "vr_trace_rays_noalloc: scratch address");
+    for (size_t i = 0; i < (size_t)n * 3; i++)
+        if (!std::isfinite(O[i]) || !std::isfinite(D[i]))
+            return fail(c, VR_ERR_ARG, "vr_trace_rays_noalloc: non-finite ray " + std::to_string(i / 3));
+    // every byte a launch reads is written by this call's copies first, and
+    // every output byte by the kernel, so nothing needs clearing
+    unsigned char* base = static_cast<unsigned char*>(c->scratch);
+    const size_t cap = VR_NOALLOC_MAX_RAYS;
+    float* dO = reinterpret_cast<float*>(base);
+    float* dD = reinterpret_cast<float*>(base + cap * 12);
+    uint32_t* dv = reinterpret_cast<uint32_t*>(base + cap * 24);
+    float* dd = reinterpret_cast<float*>(base + cap * 28);
+    uint32_t* ds = reinterpret_cast<uint32_t*>(base + cap * 32);
+    hipError_t e = hipMemcpyAsync(dO, O, (size_t)n * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dD, D, (size_t)n * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = launch_trace_batch(WorldView{c->grid, c->bricks, c->uber}, dO, dD, n, kind, steps, dv, dd, ds, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(voxel, dv, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dist, dd, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(side, ds, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(c->stream);
+        return hipfail(c, e, "vr_trace_rays_noalloc");
+    }
+    return VR_OK;
 }
 
 int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights, uint32_t nclusters)
