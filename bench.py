@@ -31,13 +31,17 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "1080p fps & Mrays/s; equal-time MSE vs converged ref, learned vs plain"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per pixel and frame of each stage (SoA planes):
+# algorithmic HBM bytes per pixel and frame of each stage (SoA planes; a
+# reservoir record is 40 B: the 16-B word the reuse tests read (M, face,
+# the pixel's hit side and distance, W, invPdf), the 16-B sample, the 8-B
+# sums):
 #   primary: write hit word, dist, seed                               12
-#   initial: read hit 12, prev hit 8, prev reservoir 36;
-#            write reservoir 36, seed 4                                96
-#   resolve (spatial reuse + shading, one kernel): read own reservoir 36
-#            + hit 12; write reservoir 36, seed 4, frame 16             104
-STAGE_BYTES = {"primary": 12, "initial": 96, "resolve": 104}
+#   initial: read hit 12, the previous frame's record at the reprojected
+#            pixel 32 (its hit side and distance, M, W, invPdf, sample;
+#            its sums are recomputed); write record 40, seed 4           88
+#   resolve (spatial reuse + shading, one kernel): read own record 40
+#            + hit 12; write record 40, seed 4, frame 16                112
+STAGE_BYTES = {"primary": 12, "initial": 88, "resolve": 112}
 # rays per pixel as the reference traces them: primary, candidate shadow ray,
 # shading shadow ray (this path looks most shading rays up instead)
 RAYS_PER_PIXEL = 3
@@ -117,8 +121,13 @@ def quality(log_fn) -> dict:
     from smallproject_amd import _lib
     from smallproject_amd.evaluate import dynamic, plain_mse
     out = {}
+    gt_cache = {}
+    # the many-light configuration as stated (softmax over clusters, uniform
+    # within the cluster, no mixture), and with the defensive mixture and
+    # the in-cluster power draw; each with both spatial merges
     for merge, flags in (("biased", 0), ("mis", _lib.FLAG_MIS_SPATIAL)):
-        out[merge] = quality_one(log_fn, merge, flags)
+        out[merge] = quality_one(log_fn, merge, flags, 0.25, _lib.CLUSTER_DRAW_POWER, gt_cache)
+        out[merge + "_as_stated"] = quality_one(log_fn, merge, flags, 0.0, _lib.CLUSTER_DRAW_UNIFORM, gt_cache)
     # the plain method's own MSE on the point-light and area-light
     # configurations (8 frames from reset history at the middle camera)
     out["plain"] = {}
@@ -144,16 +153,20 @@ def quality(log_fn) -> dict:
     return out
 
 
-def quality_one(log_fn, merge: str, flags: int) -> dict:
+def quality_one(log_fn, merge: str, flags: int, mix: float, draw: int, gt_cache: dict) -> dict:
     from smallproject_amd.evaluate import equal_time
     t0 = time.perf_counter()
-    r = equal_time("c3_clusters_1024", flags=flags)
-    log_fn(f"bench: equal-time MSE evaluation ({merge} spatial merge) in {time.perf_counter() - t0:.1f} s")
+    r = equal_time("c3_clusters_1024", flags=flags, mix=mix, draw=draw, gt_cache=gt_cache)
+    log_fn(f"bench: equal-time MSE evaluation ({merge} spatial merge, mix {mix}, {r['in_cluster_draw']} in-cluster "
+           f"draw) in {time.perf_counter() - t0:.1f} s")
     return {"workload": "c3_clusters_1024: 30,000 boxes on a floor, 65,536 emitters in 256 clusters, 1920x1080, "
                         f"M=32, {merge} spatial merge, static camera from reset history",
             "reference": f"{r['gt_spp']}-spp binary64 ground truth (its own noise MSE {r['gt_noise_mse']:.2e})",
-            "mix": r["mix"], "train": f"{r['train_frames']} frames x {r['train_steps_per_frame']} Adam steps on 2^16 "
-                                      "samples/frame from the plain method's reservoirs",
+            "mix": r["mix"], "in_cluster_draw": r["in_cluster_draw"],
+            "train": f"{r['train_frames']} path frames x {r['train_steps_per_frame']} Adam steps on 2^16 "
+                     "samples/frame from the plain method's reservoirs, then online during the comparison: "
+                     f"{r['train_steps_per_frame']} steps on 2^16 samples of each learned frame, time charged",
+            "learned_train_ms_per_frame": round(r["learned"]["stage_ms"][-1], 4),
             "plain_ms_per_frame": round(r["plain"]["ms_per_frame"], 4),
             "learned_ms_per_frame": round(r["learned"]["ms_per_frame"], 4),
             "plain_mse_8_frames": r["plain_mse_at_8"],
@@ -171,15 +184,30 @@ def bench_batch(args) -> int:
     from a hipGraph; the persistent grid size is picked by timing live
     frames.  Also times the same frames rendered view after view (one
     launch per stage and view) for comparison."""
-    from smallproject_amd import Batch, configs
-    scs = configs.small_instances() if args.config == "c4_small64" else configs.batch_instances()
+    from smallproject_amd import Batch, _lib, configs
+    scs = configs.small_instances() if args.config == "c4_small64" else \
+        configs.batch_instances(learned_every=args.learned_every)
     t0 = time.perf_counter()
     worlds = [configs.build_world(sc) for sc in scs]
-    for w in worlds:
+    learned = []
+    for sc, w in zip(scs, worlds):
         w.force_sync()
-    log(f"bench: built and uploaded {len(worlds)} worlds in {time.perf_counter() - t0:.1f} s")
+        if sc.source == _lib.SOURCE_LEARNED:
+            # each scene's own proposal: its lights in clusters, trained online
+            cl = w.cluster_lights(sc.clusters, 8, 1)
+            w.renderer.set_light_clusters(cl, int(cl.max()) + 1)
+            learned.append(w)
+    log(f"bench: built and uploaded {len(worlds)} worlds in {time.perf_counter() - t0:.1f} s "
+        f"({len(learned)} with a learned proposal)")
     npx = sum(sc.width * sc.height for sc in scs)
     frame_no = [0]
+
+    def finetune():
+        # every 4th frame each learned view takes one Adam step on 2^14
+        # samples of its own last frame (the time counts in the batch's)
+        if frame_no[0] % 4 == 3:
+            for w in learned:
+                w.renderer.mlp_train(1 << 14, 1e-3, 1)
 
     def place(f):
         for sc, w in zip(scs, worlds):
@@ -191,6 +219,7 @@ def bench_batch(args) -> int:
             place(frame_no[0])
             for w in worlds:
                 w.render()
+            finetune()
             frame_no[0] += 1
         for w in worlds:
             w.renderer.sync()
@@ -202,6 +231,7 @@ def bench_batch(args) -> int:
         for _ in range(nframes):
             place(frame_no[0])
             b.render()
+            finetune()
             frame_no[0] += 1
         b.sync()
 
@@ -241,8 +271,12 @@ def bench_batch(args) -> int:
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural worlds, lights and camera paths)",
         "config": {"workload": f"batch of {len(scs)} views, {npx} pixels per batch frame, area lights, M=32, "
                                "5 taps r=30, temporal clamp 20x",
-                   "views": [{"world": sc.world_size, "resolution": [sc.width, sc.height], "lights": len(w.lights())}
+                   "views": [{"world": sc.world_size, "resolution": [sc.width, sc.height], "lights": len(w.lights()),
+                              "source": "learned" if sc.source == _lib.SOURCE_LEARNED else "uniform"}
                              for sc, w in zip(scs, worlds)],
+                   "learned_views": len(learned),
+                   "learned_training": "each learned view: 1 Adam step on 2^14 samples of its own frame every 4 "
+                                       "frames, inside the timed frames" if learned else None,
                    "graph": not args.no_graph, "views_per_chunk": best[0], "blocks": best[1]},
         "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
         "gpu_ms_per_batch_frame": round(gpu_ms / max(nprof, 1), 4),
@@ -271,6 +305,8 @@ def main() -> int:
     ap.add_argument("--spatial", default="biased", choices=["biased", "mis"],
                     help="spatial merge: the reference's (biased) or MIS-weighted with a visibility ray per neighbour")
     ap.add_argument("--no-graph", action="store_true", help="batched views: launch without the hipGraph replay")
+    ap.add_argument("--learned-every", type=int, default=1,
+                    help="batched views: every k-th view samples from its own scene's learned proposal (0: none)")
     ap.add_argument("--calibrate", action="store_true",
                     help="run only the known-size copy used to scale HBM byte counters")
     args = ap.parse_args()
@@ -332,9 +368,40 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     stage_sum, nprof = r.profile_read()
     log(f"bench: {args.steps} timed frames in {elapsed:.3f} s")
+    # rays actually traced, counted on the device over a few frames
+    # (every pixel's primary ray; a candidate shadow ray where a candidate
+    # has weight; a shading ray only where the candidate pass did not trace
+    # the same ray; MIS neighbour rays likewise)
+    count_frames = 8
+    r.set_ray_counting(True)
+    for f in range(frames_total, frames_total + count_frames):
+        step(f)
+    rays = r.ray_counts()
+    r.set_ray_counting(False)
+    traced_per_frame = sum(rays.values()) / count_frames
+    log(f"bench: rays traced per frame {traced_per_frame / 1e6:.3f} M " +
+        ", ".join(f"{k} {v / count_frames / 1e6:.3f} M" for k, v in rays.items()))
+    # the whole camera path once, frames 0 .. frames-1 in order (the timed
+    # steps above cover frames warmup .. warmup+steps-1 of it, wrapping)
+    path_frames = sc.frames if not dynamic else 0
+    path_elapsed = None
+    if path_frames:
+        step(0)
+        r.sync()
+        t1 = time.perf_counter()
+        for f in range(1, path_frames):
+            step(f)
+        r.sync()
+        path_elapsed = time.perf_counter() - t1
+        log(f"bench: frames 1-{path_frames - 1} of the path in {path_elapsed:.3f} s")
 
     npx = sc.width * sc.height
     fps = args.steps / elapsed
+    grid = w.grid()
+    occupancy = {"top_cells": int(grid.size), "brick_cells": int(np.count_nonzero(grid & 1)),
+                 "solid_cells": int(np.count_nonzero((grid != 0) & ((grid & 1) == 0)))}
+    occupancy["brick_fraction"] = round(occupancy["brick_cells"] / grid.size, 5)
+    occupancy["nonempty_fraction"] = round((occupancy["brick_cells"] + occupancy["solid_cells"]) / grid.size, 5)
     stage_ms = {name: float(stage_sum[i] / max(nprof, 1)) for i, name in enumerate(_lib.STAGE_NAMES)}
     dom = max(stage_ms, key=stage_ms.get)
     achieved = STAGE_BYTES[dom] * npx / (stage_ms[dom] * 1e-3) / 1e9
@@ -364,10 +431,19 @@ def main() -> int:
                    "tile_order": tile_order,
                    "tile_order_stage_ms": None if tune_ms is None else
                    {k: [round(v, 4) for v in ms] for k, ms in tune_ms.items()},
+                   "world_occupancy": occupancy,
                    "waves_per_simd": waves,
                    "waves_stage_ms": None if waves_ms is None else
                    {("default" if k == 0 else str(k)): [round(v, 4) for v in ms] for k, ms in waves_ms.items()}},
         "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),
+        "mrays_per_s_note": f"{RAYS_PER_PIXEL} rays per pixel, as the reference traces them (primary, candidate "
+                            "shadow ray, shading shadow ray)",
+        "traced_mrays_per_s": round(fps * traced_per_frame / 1e6, 1),
+        "traced_rays_per_pixel": {k: round(v / count_frames / npx, 4) for k, v in rays.items()},
+        "path": None if not path_elapsed else {
+            "frames": path_frames - 1, "fps": round((path_frames - 1) / path_elapsed, 3),
+            "ms_per_frame": round(1000.0 * path_elapsed / (path_frames - 1), 4),
+            "note": "frames 1..N-1 of the camera path in order after frame 0, the whole path's rate"},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,

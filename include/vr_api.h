@@ -5,8 +5,8 @@
  * the reference: the OpenCL Kernel/Buffer runtime (template/template.cpp:
  * 1266-1754) and the ReSTIR kernels of cl/ris.cl, cl/trace.cl, cl/tools.cl,
  * cl/phat.cl, cl/reservoir.cl, cl/spatialSampling.cl and the commit/ubergrid
- * kernels of cl/kernels.cl.  Above it, the C++ World class (world.h in the
- * package) keeps the reference's host API and calls only these functions.
+ * kernels of cl/kernels.cl.  Above it, the C++ World class (world.h beside
+ * this header) keeps the reference's host API and calls only these functions.
  *
  * Conventions: every function returns VR_OK (0) or a VR_ERR_* code and never
  * throws; vr_last_error() describes the last failure of a context.  Pointers
@@ -248,6 +248,18 @@ int vr_read_frame(vr_ctx* ctx, float* rgba);
  * next to that stage. */
 int vr_stage_times(vr_ctx* ctx, float ms[VR_NUM_STAGES]);
 
+/* Rays traced by the following frames (a diagnostic with no counterpart in
+ * the reference, which traces 3 per pixel: primary, candidate and shading
+ * shadow rays).  vr_set_ray_counting(ctx, 1) zeroes the counters and counts
+ * from the next frame on, with one atomic add per traced ray; 0 stops.
+ * vr_read_ray_counts waits for the queued frames and returns the sums:
+ * [0] primary rays (with renderAlbedo's translucency continuations),
+ * [1] the candidate pass's shadow rays, [2] shading's (those the candidate
+ * pass had not traced), [3] the MIS merge's neighbour visibility rays. */
+#define VR_RAY_KINDS 4
+int vr_set_ray_counting(vr_ctx* ctx, int enable);
+int vr_read_ray_counts(vr_ctx* ctx, uint64_t counts[VR_RAY_KINDS]);
+
 /* Stage profiling over many frames (the reference keeps 100-frame averages,
  * template/world.cpp:2342-2353): vr_profile_begin starts summing every
  * frame's per-stage event times; vr_profile_read waits for the profiled
@@ -291,6 +303,15 @@ int vr_trace_rays_noalloc(vr_ctx* ctx, const float* origins, const float* dirs, 
 int vr_set_light_clusters(vr_ctx* ctx, const uint32_t* cluster_of_light, uint32_t nlights, uint32_t nclusters);
 /* eps of the defensive mixture, 0..1 (default 0.25) */
 int vr_set_proposal_mix(vr_ctx* ctx, float eps);
+/* how the light inside a drawn cluster is chosen: in proportion to its
+ * power (VR_CLUSTER_DRAW_POWER, the default; source pdf m_c * pmf_i / P_c)
+ * or uniformly (VR_CLUSTER_DRAW_UNIFORM; source pdf m_c / n_c, the
+ * many-light configuration's "uniform within the cluster").  Takes effect
+ * with the next vr_set_light_clusters, which must follow before a learned
+ * frame. */
+#define VR_CLUSTER_DRAW_POWER   0u
+#define VR_CLUSTER_DRAW_UNIFORM 1u
+int vr_set_cluster_draw(vr_ctx* ctx, uint32_t mode);
 /* the cluster tables (any pointer may be NULL): start[257] offsets into
  * lights[n] (light ids grouped by cluster), per-cluster power alias tables
  * prob[n] / alias[n] (cluster-local indices) aligned with lights, share[n]

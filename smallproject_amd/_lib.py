@@ -26,6 +26,7 @@ TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS = 0, 1, 2
 FLAG_MIS_SPATIAL, MIS_MAX_TAPS = 1 << 8, 32
 MAX_VIEWS = 64
 NOALLOC_MAX_RAYS = 1 << 18  # VR_NOALLOC_MAX_RAYS
+CLUSTER_DRAW_POWER, CLUSTER_DRAW_UNIFORM = 0, 1
 NUM_STAGES = 3
 STAGE_NAMES = ("primary", "initial", "resolve")  # resolve = spatial reuse + shading
 GRIDSIZE = 128 ** 3
@@ -80,6 +81,9 @@ SIGNATURES = {
     "vr_profile_read": (C.c_int, [_P, _P, _P]),
     "vr_trace_rays": (C.c_int, [_P, _P, _P, _u32, _u32, _i32, _P, _P, _P]),
     "vr_trace_rays_noalloc": (C.c_int, [_P, _P, _P, _u32, _u32, _i32, _P, _P, _P]),
+    "vr_set_ray_counting": (C.c_int, [_P, C.c_int]),
+    "vr_set_cluster_draw": (C.c_int, [_P, _u32]),
+    "vr_read_ray_counts": (C.c_int, [_P, _P]),
     "vr_ground_truth": (C.c_int, [_P, _P, _u32, _u32, _P]),
     "vr_set_light_clusters": (C.c_int, [_P, _P, _u32, _u32]),
     "vr_mlp_set_params": (C.c_int, [_P, _P, _u32]),

@@ -199,6 +199,11 @@ class Renderer:
         self._chk(lib().vr_mlp_train(self.h, src, nsamples, C.c_float(lr), steps, C.byref(loss)), "vr_mlp_train")
         return loss.value
 
+    def set_cluster_draw(self, mode: int):
+        """Light inside a drawn cluster: by power (CLUSTER_DRAW_POWER) or
+        uniform (CLUSTER_DRAW_UNIFORM); call set_light_clusters after it."""
+        self._chk(lib().vr_set_cluster_draw(self.h, mode), "vr_set_cluster_draw")
+
     def set_proposal_mix(self, eps: float):
         self._chk(lib().vr_set_proposal_mix(self.h, C.c_float(eps)), "vr_set_proposal_mix")
 
@@ -217,6 +222,18 @@ class Renderer:
         pdf = np.zeros((_lib.CANDIDATES, self.n), np.float32)
         self._chk(lib().vr_read_candidates(self.h, ptr(idx), ptr(pdf)), "vr_read_candidates")
         return idx, pdf
+
+    def set_ray_counting(self, enable: bool):
+        """Count the rays the following frames trace (zeroes the counters)."""
+        self._chk(lib().vr_set_ray_counting(self.h, int(enable)), "vr_set_ray_counting")
+
+    def ray_counts(self) -> dict:
+        """Rays traced since counting started: primary (with translucency
+        continuations), candidate shadow rays, shading shadow rays, MIS
+        neighbour rays."""
+        v = np.zeros(4, np.uint64)
+        self._chk(lib().vr_read_ray_counts(self.h, ptr(v)), "vr_read_ray_counts")
+        return dict(zip(("primary", "candidate", "shading", "mis"), (int(x) for x in v)))
 
     def trace(self, O: np.ndarray, D: np.ndarray, kind: int = _lib.TRACE_ALL, steps: int = 999999):
         O = np.ascontiguousarray(O, dtype=np.float32).reshape(-1, 3)

@@ -19,6 +19,8 @@ and render settings.  `build_world` constructs the host World (CPU only);
                  the device, MIS-weighted spatial merge, 120 frames
   c1_translucent_128  configuration 1's scene with 400 translucent panes
                  (alpha 3..12 of 15) over the terrain
+  letters_1600   the reference's letters scene file (1249 emissive voxels)
+                 from its recorded camera, 1600x1024, area lights
   batch_instances()  16 scene instances of uneven world size (256^3 -
                  1024^3) and resolution (320x180 - 1920x1080) rendered as one
                  batch of views
@@ -30,8 +32,16 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass, field
 
+from pathlib import Path
+
+import numpy as np
+
 from . import _lib
 from .api import World, camera_path, default_config, focal_for_fov, log_lut
+
+# the reference's own scene files (scene_export/<scene>/grid.vx), kept as
+# test fixtures
+SCENE_DIR = Path(__file__).resolve().parent.parent / "tests" / "golden"
 
 
 @dataclass
@@ -71,6 +81,10 @@ class SceneConfig:
     # scene motion on the device (vr_scene_step): light speeds uniform in
     # [0, light_speed] voxels/frame in uniform directions (motion_seed);
     # edit_permille of the bricks edited per frame by walk_steps-step walks
+    # a scene file (.vx, stamped at the origin) instead of a generated world,
+    # and a fixed camera (position, target) instead of a path
+    vx: str | None = None
+    fixed_camera: tuple | None = None
     # translucent panes (alpha 3..12 of 15) over the terrain, pane_seed
     panes: int = 0
     pane_seed: int = 0
@@ -87,6 +101,9 @@ class SceneConfig:
                               source=self.source, lut=lut, flags=self.flags, capacity=self.capacity)
 
     def camera(self, frame: int):
+        if self.fixed_camera is not None:
+            pos, tgt = self.fixed_camera
+            return np.asarray(pos, np.float32), np.asarray(tgt, np.float32)
         return camera_path(self.camera_kind, self.camera_seed, frame, self.frames, self.world_size, self.scale,
                            self.terrain_seed)
 
@@ -111,6 +128,15 @@ CONFIGS = {
         light_mode="surface", emit_range=(1, 255), lut=(0.1, 10.0), camera_kind=1, camera_seed=6,
         flags=_lib.FLAG_MIS_SPATIAL, light_speed=3.0, motion_seed=9, edit_permille=50, walk_steps=32,
         edit_seed=10),
+    # the reference's letters scene (scene_export/letters/grid.vx: 1249
+    # emissive voxels) from its recorded first camera angle
+    # (scene_experiment_info.txt:9-11) at the reference's resolution and
+    # focal distance 2.2 (template/common.h:2-3, world.cpp:2155-2157), area
+    # lights, M=32: the setting of results.xlsx's "letters (1)" row
+    "letters_1600": SceneConfig(
+        name="letters_1600", width=1600, height=1024, point=False, fov=2.0 * math.degrees(math.atan(1.0 / 2.2)),
+        frames=60, world_size=1024, scale=0, terrain_seed=0, vx="letters_grid.vx",
+        fixed_camera=((152.32, 59.82, -49.48), (152.32 + 0.11, 59.82 - 0.22, -49.48 + 0.97))),
     # configuration 1 with 400 translucent panes: renderAlbedo's
     # compositing loop and shadow rays through translucent voxels
     "c1_translucent_128": SceneConfig(
@@ -229,7 +255,9 @@ def build_world(sc: SceneConfig | str, width: int | None = None, height: int | N
     if isinstance(sc, str):
         sc = CONFIGS[sc]
     w = World(sc.config(width, height))
-    if sc.floor:
+    if sc.vx:
+        w.load_vx(SCENE_DIR / sc.vx)
+    elif sc.floor:
         w.scene_floor_boxes(sc.boxes, sc.box_edge[0], sc.box_edge[1], sc.world_size, sc.floor, sc.box_seed)
     else:
         w.scene_terrain(sc.world_size, sc.scale, sc.terrain_seed)
@@ -239,7 +267,7 @@ def build_world(sc: SceneConfig | str, width: int | None = None, height: int | N
     if sc.panes:
         add_panes(w, sc)
     w.optimize()
-    if sc.lights:
+    if sc.lights and not sc.vx:
         if sc.light_mode == "clusters":
             w.scene_light_clusters(sc.lights, sc.clusters, sc.cluster_sigma, sc.world_size, sc.cluster_y[0],
                                    sc.cluster_y[1], sc.light_seed, sc.emit_range[0], sc.emit_range[1])
@@ -248,7 +276,7 @@ def build_world(sc: SceneConfig | str, width: int | None = None, height: int | N
                                  sc.light_height[0], sc.light_height[1], sc.emit_range[0], sc.emit_range[1])
         else:
             w.scene_lights_surface(sc.lights, sc.world_size, sc.light_seed, sc.emit_range[0], sc.emit_range[1])
-    w.set_focal(focal_for_fov(sc.fov))
+    w.set_focal(2.2 if sc.vx else focal_for_fov(sc.fov))
     p = w.params
     p.numberOfCandidates, p.spatialTaps, p.spatialRadius = sc.candidates, sc.taps, sc.radius
     p.numberOfMaxTemporalImportance = sc.temporal_clamp

@@ -353,15 +353,18 @@ struct Res {
     uint32_t face;  // Nlight as face index, 7 = zero vector
 };
 
-__device__ __forceinline__ void res_update(Res& r, float w, float pHat, uint32_t index, float r0, f3 pos,
+// true when the new sample takes the reservoir's place
+__device__ __forceinline__ bool res_update(Res& r, float w, float pHat, uint32_t index, float r0, f3 pos,
                                            float invPdf, uint32_t face)
 {
     // UpdateReservoir (reservoir.cl:3-15)
     r.M += 1u;
     r.wsum += w;
-    if (r0 < w / fmaxf(1e-5f, r.wsum)) {
+    const bool take = r0 < w / fmaxf(1e-5f, r.wsum);
+    if (take) {
         r.light = index, r.pos = pos, r.invPdf = invPdf, r.face = face, r.pHat = pHat;
     }
+    return take;
 }
 
 __device__ __forceinline__ void res_reweigh(Res& r, float newPHat, uint32_t M)
@@ -372,12 +375,14 @@ __device__ __forceinline__ void res_reweigh(Res& r, float newPHat, uint32_t M)
     r.pHat = newPHat;
 }
 
-__device__ __forceinline__ void res_combine(Res& r, const Res& o, float r0)
+// true when o's sample takes the reservoir's place
+__device__ __forceinline__ bool res_combine(Res& r, const Res& o, float r0)
 {
     // CombineReservoir (reservoir.cl:30-38)
     const uint32_t m1 = r.M;
-    res_update(r, o.wsum, o.pHat, o.light, r0, o.pos, o.invPdf, o.face);
+    const bool take = res_update(r, o.wsum, o.pHat, o.light, r0, o.pos, o.invPdf, o.face);
     r.M = m1 + o.M;
+    return take;
 }
 
 __device__ __forceinline__ void res_adjust(Res& r, float pHat)

@@ -10,11 +10,26 @@ struct HitPlanes {
     uint32_t* seed;
 };
 
+// A plane of per-pixel reservoirs, as three structure-of-arrays records.
+// r0 holds what a reuse pass tests before it takes a reservoir, so a
+// neighbour or the previous frame's pixel that fails the tests costs one
+// 16-byte read: {M (bits 0-23) | Nlight face << 24 | the pixel's hit side
+// bits << 27 (side and the sign of D[side], bits 24-26 of the hit word) |
+// kTraced | kVisible, hit distance, W, invPdf}.  r1 is the sample {pos.xyz,
+// light index}, r2 the sums {wsum, pHat}.
 struct ResPlanes {
-    float4* a;    // wsum, W, pHat, invPdf
-    float4* b;    // pos.xyz, lightIndex bits
-    uint32_t* m;  // M | Nlight face << 24
+    uint4* r0;
+    uint4* r1;
+    float2* r2;
 };
+// r0.x bits 30-31: the candidate pass traced the shadow ray of this
+// reservoir's sample from this pixel (kTraced), and found it unoccluded
+// (kVisible).  A later ray from the same surface point to the same sample
+// is the same ray with the same answer.
+constexpr uint32_t kTraced = 1u << 30, kVisible = 1u << 31;
+
+// ray counts of a frame (FrameArgs::rays, when counting is on)
+enum RayCount { kRaysPrimary = 0, kRaysCandidate = 1, kRaysShading = 2, kRaysMis = 3, kRayKinds = 4 };
 
 // Everything one frame's kernels read, passed by value (kernarg segment).
 struct FrameArgs {
@@ -22,12 +37,9 @@ struct FrameArgs {
     LightView L;
     Camera cam;
     HitPlanes cur;
-    const uint32_t* prevHitw;
-    const float* prevDist;
     ResPlanes A;  // final reservoirs: previous frame's on entry, this frame's on exit
     ResPlanes B;  // candidates + temporal output
-    uint4* nbr;   // [2 per pixel] the B reservoir as the spatial pass's neighbours read it
-    uint4* tested; // [per pixel] the candidate pass's shadow ray: {light | visible << 31, pos bits}
+    unsigned long long* rays;  // [kRayKinds] rays traced, or nullptr (not counted)
     float4* frame;
     const float* emitLut;
     uint32_t W, H;

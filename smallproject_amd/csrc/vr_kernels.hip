@@ -11,7 +11,8 @@
 // pass of the same thread has just written (no other pixel's spatial pass
 // reads it), so the two run as one kernel.
 // Per-pixel state lives in structure-of-arrays planes (12 B of hit data and
-// 36 B per reservoir) so every pass streams coalesced 4/16-byte words.
+// 40 B per reservoir record: 16 + 16 + 8) so every pass streams coalesced
+// 4/8/16-byte words.
 #include "vr_kernels.h"
 
 namespace vr {
@@ -25,56 +26,49 @@ __device__ __forceinline__ void tile_pixel(uint32_t order, int& x, int& y)
     y = ty * 16 + (wv >> 1) * 8 + (lane >> 3);
 }
 
-__device__ __forceinline__ Res load_res(const ResPlanes& r, size_t i)
+__device__ __forceinline__ Res res_from(const uint4 r0, const uint4 r1, const float2 r2)
 {
-    const float4 a = r.a[i], b = r.b[i];
-    const uint32_t m = r.m[i];
     Res o;
-    o.wsum = a.x, o.W = a.y, o.pHat = a.z, o.invPdf = a.w;
-    o.pos = f3{b.x, b.y, b.z};
-    o.light = __float_as_uint(b.w);
-    o.M = m & 0xffffffu;
-    o.face = m >> 24;
+    o.M = r0.x & 0xffffffu;
+    o.face = (r0.x >> 24) & 7u;
+    o.W = __uint_as_float(r0.z), o.invPdf = __uint_as_float(r0.w);
+    o.pos = f3{__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z)};
+    o.light = r1.w;
+    o.wsum = r2.x, o.pHat = r2.y;
     return o;
 }
 
-__device__ __forceinline__ void store_res(const ResPlanes& r, size_t i, const Res& o)
+__device__ __forceinline__ Res load_res(const ResPlanes& r, size_t i) { return res_from(r.r0[i], r.r1[i], r.r2[i]); }
+
+// the record of reservoir o at the pixel with hit word hitw and distance
+// dist; shadow: kTraced / kVisible
+__device__ __forceinline__ void store_res(const ResPlanes& r, size_t i, const Res& o, uint32_t hitw, float dist,
+                                          uint32_t shadow)
 {
-    r.a[i] = make_float4(o.wsum, o.W, o.pHat, o.invPdf);
-    r.b[i] = make_float4(o.pos.x, o.pos.y, o.pos.z, __uint_as_float(o.light));
-    r.m[i] = (o.M & 0xffffffu) | (o.face << 24);
+    r.r0[i] = make_uint4((o.M & 0xffffffu) | ((o.face & 7u) << 24) | (((hitw >> 24) & 7u) << 27) | shadow,
+                         __float_as_uint(dist), __float_as_uint(o.W), __float_as_uint(o.invPdf));
+    r.r1[i] = make_uint4(__float_as_uint(o.pos.x), __float_as_uint(o.pos.y), __float_as_uint(o.pos.z), o.light);
+    r.r2[i] = make_float2(o.wsum, o.pHat);
 }
 
-// The spatial pass's view of a neighbour, written by k_initial after its
-// reservoir is final: {M | face << 24 | hit side bits << 27, dist, W,
-// invPdf} {pos.xyz, light}.  A tap then reads one 32-byte record instead of
-// the hit word, distance and three reservoir planes.
-__device__ __forceinline__ void store_neighbour(uint4* nbr, size_t i, const Res& r, uint32_t hitw, float dist)
-{
-    nbr[2 * i] = make_uint4((r.M & 0xffffffu) | ((r.face & 7u) << 24) | (((hitw >> 24) & 7u) << 27),
-                            __float_as_uint(dist), __float_as_uint(r.W), __float_as_uint(r.invPdf));
-    nbr[2 * i + 1] = make_uint4(__float_as_uint(r.pos.x), __float_as_uint(r.pos.y), __float_as_uint(r.pos.z), r.light);
-}
+// a record word's hit side bits in hit-word position (bits 24-26)
+__device__ __forceinline__ uint32_t rec_hit_bits(uint32_t w) { return (w >> 3) & (7u << 24); }
 
-// The shadow ray k_initial traced for its candidate sample: {light |
-// visible << 31, pos.xyz bits}, or kNoTest.  A later shadow ray from the same
-// surface point to the same light and point is the same ray with the same
-// answer, so shading and the MIS merge look it up instead of tracing again.
-constexpr uint32_t kNoTest = 0xffffffffu;
-
-__device__ __forceinline__ uint4 tested_record(uint32_t light, f3 pos, bool visible)
+// Visibility of sample (light, pos) from a record's pixel as the candidate
+// pass found it: 1 = visible, 0 = occluded, -1 = not traced (the record's
+// sample is another one, or its ray was not traced).
+__device__ __forceinline__ int known_visibility(uint32_t w0, const uint4 r1, uint32_t light, f3 pos)
 {
-    return make_uint4(light | (visible ? 0x80000000u : 0u), __float_as_uint(pos.x), __float_as_uint(pos.y),
-                      __float_as_uint(pos.z));
-}
-
-// 1 = visible, 0 = occluded, -1 = not traced for this sample
-__device__ __forceinline__ int tested_visibility(const uint4 t, uint32_t light, f3 pos)
-{
-    if (t.x == kNoTest || (t.x & 0x7fffffffu) != light || t.y != __float_as_uint(pos.x) ||
-        t.z != __float_as_uint(pos.y) || t.w != __float_as_uint(pos.z))
+    if (!(w0 & kTraced) || r1.w != light || r1.x != __float_as_uint(pos.x) || r1.y != __float_as_uint(pos.y) ||
+        r1.z != __float_as_uint(pos.z))
         return -1;
-    return (int)(t.x >> 31);
+    return (w0 & kVisible) ? 1 : 0;
+}
+
+// rays of kind k traced by this lane (only while a frame's rays are counted)
+__device__ __forceinline__ void count_rays(unsigned long long* rays, int k, uint32_t n)
+{
+    if (rays && n) atomicAdd(&rays[k], (unsigned long long)n);
 }
 
 // GridCoordinatesFromHit + Uint3CubeEquals (cl/tools.cl:452-472) behind
@@ -132,12 +126,29 @@ __device__ __forceinline__ f3 face_point(f3 c, float hs, float size, int a, floa
 __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t& seed, f3& pos, float& invPdf,
                                                uint32_t& face)
 {
-    uint32_t u[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) u[k] = xorshift(seed);
-    const float r2 = rnd(seed);
     const float hs = 0.5f * size;
     const float area = size * size;
+    // the face of each axis that can face the point is known before the
+    // draws: the twelve numbers are drawn in face order and each axis keeps
+    // only the pair of its front face (of its -face when neither faces it)
+    bool plusA[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float ca = a == 0 ? c.x : (a == 1 ? c.y : c.z);
+        const float spa = a == 0 ? sp.x : (a == 1 ? sp.y : sp.z);
+        plusA[a] = (ca + hs) - spa < 0.0f;
+    }
+    float q0[3], q1[3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const uint32_t v0 = xorshift(seed), v1 = xorshift(seed);
+        const int a = k % 3;
+        if (plusA[a] == (k < 3)) {
+            q0[a] = (float)v0 * 2.3283064365387e-10f;
+            q1[a] = (float)v1 * 2.3283064365387e-10f;
+        }
+    }
+    const float r2 = rnd(seed);
     float sol[3];
     int fidx[3];   // front face of each axis, or -1
     f3 fp0, fp1, fp2;  // the point drawn on that face (named: an array would be indexed at run time below)
@@ -146,12 +157,10 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
         const float ca = a == 0 ? c.x : (a == 1 ? c.y : c.z);
         const float spa = a == 0 ? sp.x : (a == 1 ? sp.y : sp.z);
         const float Rp = (ca + hs) - spa, Rm = (ca - hs) - spa;
-        const bool plus = Rp < 0.0f, minus = Rm > 0.0f;
+        const bool plus = plusA[a], minus = Rm > 0.0f;
         const float s = plus ? 1.0f : -1.0f;
         const int i = plus ? a : a + 3;
-        const float r0 = (float)(plus ? u[2 * a] : u[2 * a + 6]) * 2.3283064365387e-10f;
-        const float r1 = (float)(plus ? u[2 * a + 1] : u[2 * a + 7]) * 2.3283064365387e-10f;
-        const f3 fpa = face_point(c, hs, size, a, s, r0, r1);
+        const f3 fpa = face_point(c, hs, size, a, s, q0[a], q1[a]);
         if (a == 0) fp0 = fpa;
         else if (a == 1) fp1 = fpa;
         else fp2 = fpa;
@@ -263,6 +272,7 @@ __device__ __forceinline__ void primary_px(const FrameArgs& a, int x, int y)
             total += dist + 0.01f;
             const f3 O = f3{a.cam.E.x + total * D.x, a.cam.E.y + total * D.y, a.cam.E.z + total * D.z};
             voxel = trace<kTraceThrough>(a.w, O, D, 999999, voxel, dist, side);
+            count_rays(a.rays, kRaysPrimary, 1u);
             c = to_rgb(voxel);
             alpha = alphaf(voxel);
             out.x += remaining * c.x * alpha, out.y += remaining * c.y * alpha, out.z += remaining * c.z * alpha;
@@ -279,12 +289,29 @@ __device__ __forceinline__ void primary_px(const FrameArgs& a, int x, int y)
     a.cur.hitw[idx] = enc | (side << 24) | ((ds > 0 ? 1u : 0u) << 26);
     a.cur.dist[idx] = total;
     a.cur.seed[idx] = seed;
+    count_rays(a.rays, kRaysPrimary, 1u);
 }
 
 // shading point set-up shared by the reuse passes
 struct Shading {
     f3 D, N, brdf, sp;
 };
+
+// VoxelNormal (cl/tools.cl:416-421) from a hit word: side bits 24-25, the
+// sign of D[side] in bit 26
+__device__ __forceinline__ f3 hit_normal(uint32_t hitw)
+{
+    const uint32_t side = (hitw >> 24) & 3u;
+    const float s = (hitw >> 26) & 1u ? -1.0f : 1.0f;
+    return f3{side == 0 ? s : 0.0f, side == 1 ? s : 0.0f, side == 2 ? s : 0.0f};
+}
+
+// ToFloatRGB(voxel) * INVPI (cl/ris.cl:48)
+__device__ __forceinline__ f3 hit_brdf(uint32_t hitw)
+{
+    const f3 c = to_rgb(hitw);
+    return f3{c.x * kInvPi, c.y * kInvPi, c.z * kInvPi};
+}
 
 __device__ __forceinline__ Shading shading_at(const FrameArgs& a, int x, int y, uint32_t hitw, float dist)
 {
@@ -308,7 +335,8 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
     Res res;
     res.wsum = 0.0f, res.W = 0.0f, res.pHat = 0.0f, res.invPdf = 0.0f;
     res.pos = f3{0.0f, 0.0f, 0.0f}, res.light = 0u, res.M = 0u, res.face = 7u;
-    uint4 tested = make_uint4(kNoTest, 0u, 0u, 0u);
+    uint32_t shadow = 0u;  // kTraced | kVisible of the B sample's ray
+    uint32_t traced = 0u;  // a shadow ray was traced
     if ((hitw & 0xffffffu) != 0u) {
         uint32_t seed = a.cur.seed[idx];
         const Shading s = shading_at(a, x, y, hitw, dist);
@@ -362,9 +390,19 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
             const f3 L = normalize(sub(res.pos, spo));
             const bool occ = light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
             if (occ) res.W = 0.0f, res.wsum = 0.0f;
-            tested = tested_record(res.light, res.pos, !occ);
+            shadow = kTraced | (occ ? 0u : kVisible);
+            traced = 1u;
         }
         if (a.restirtemporalframe > 0u && a.temporal) {
+            // The normal and brdf again from the hit word (the same bits:
+            // the word carries the side and the sign of D[side], and the
+            // colour): nothing but the shading point stays live through the
+            // shadow ray's traversal.  The empty asm keeps the compiler from
+            // reusing the values computed before it.
+            uint32_t hw = hitw;
+            asm volatile("" : "+v"(hw));
+            const Shading s2 = Shading{s.D, hit_normal(hw), hit_brdf(hw), s.sp};
+            const Shading& s = s2;
             // ReprojectWorldPoint (cl/tools.cl:479-488)
             const float dl = dot(s.sp, f3{a.Nl.x, a.Nl.y, a.Nl.z}) - a.Nl.w;
             const float dr = dot(s.sp, f3{a.Nr.x, a.Nr.y, a.Nr.z}) - a.Nr.w;
@@ -373,19 +411,24 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
             const int px = f2i(roundf((float)a.W * (dl / (dl + dr))));
             const int py = f2i(roundf((float)a.H * (dt / (dt + db))));
             if (px >= 0 && py >= 0 && px < (int)a.W && py < (int)a.H) {
+                // the previous frame's final reservoir at the reprojected
+                // pixel: its record word and distance carry that pixel's
+                // hit (the previous hit buffer's side bits and distance)
                 const size_t pi = (size_t)py * a.W + px;
-                const uint32_t ph = a.prevHitw[pi];
-                const float pd = a.prevDist[pi];
-                if (same_face(ph, hitw, a.axisOnlyFace) && fabsf(dist - pd) <= 0.1f * dist) {
-                    Res prev = load_res(a.A, pi);
-                    const uint32_t cap = a.maxTemporal * nC;
-                    const uint32_t pm = cap < prev.M ? cap : prev.M;
-                    if (pm > 0u && prev.light < nL) {
+                const uint4 p0 = a.A.r0[pi];
+                const float pd = __uint_as_float(p0.y);
+                const uint32_t prevM = p0.x & 0xffffffu;
+                const uint32_t cap = a.maxTemporal * nC;
+                const uint32_t pm = cap < prevM ? cap : prevM;
+                if (same_face(rec_hit_bits(p0.x), hitw, a.axisOnlyFace) && fabsf(dist - pd) <= 0.1f * dist && pm > 0u) {
+                    const uint4 p1 = a.A.r1[pi];
+                    Res prev = res_from(p0, p1, make_float2(0.0f, 0.0f));  // wsum and pHat are recomputed below
+                    if (prev.light < nL) {
                         const float4 pa = a.L.a[prev.light], pb = a.L.b[prev.light];
                         const float pph = phat(a.point, prev.pos, face_normal(prev.face), s.sp, s.N, s.brdf,
                                                f3{pb.x, pb.y, pb.z}, pa.w) * prev.invPdf;
                         res_reweigh(prev, pph, pm);
-                        res_combine(res, prev, rnd(seed));
+                        if (res_combine(res, prev, rnd(seed))) shadow = 0u;  // the traced sample was replaced
                         const float4 ra = a.L.a[res.light], rb = a.L.b[res.light];
                         const float rph = phat(a.point, res.pos, face_normal(res.face), s.sp, s.N, s.brdf,
                                                f3{rb.x, rb.y, rb.z}, ra.w) * res.invPdf;
@@ -396,9 +439,8 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
         }
         a.cur.seed[idx] = seed;
     }
-    store_res(a.B, idx, res);
-    store_neighbour(a.nbr, idx, res, hitw, dist);
-    a.tested[idx] = tested;
+    store_res(a.B, idx, res, hitw, dist, shadow);
+    count_rays(a.rays, kRaysCandidate, traced);
 }
 
 // ---------------------------------------------------------------------------
@@ -420,11 +462,13 @@ __device__ __forceinline__ void tap_offset(uint32_t& seed, float rad, int& dx, i
 // it is merged with the accepted taps and adjusted, `seed` is the stream
 // after the taps, and the MIS merge's bookkeeping is kept: M0 (own M),
 // seed0 (the stream before the taps), `mask` (combined taps) and `sel`
-// (tap whose sample was taken last, -1 = own).
+// (tap whose sample was taken last, -1 = own).  `shadow` is the own B
+// record's kTraced | kVisible while its sample stays the merged one, else 0.
 struct Merge {
     Res res;
     Shading s;
-    uint32_t hitw, seed, seed0, M0, mask;
+    uint32_t hitw, seed, seed0, M0, mask, shadow;
+    float dist;
     int sel;
     bool merged;
 };
@@ -434,13 +478,16 @@ __device__ __forceinline__ Merge spatial_merge(const FrameArgs& a, int x, int y)
 {
     Merge g;
     const size_t idx = (size_t)y * a.W + x;
-    g.res = load_res(a.B, idx);
+    const uint4 o0 = a.B.r0[idx];
+    g.res = res_from(o0, a.B.r1[idx], a.B.r2[idx]);
+    g.shadow = o0.x & (kTraced | kVisible);
     g.hitw = a.cur.hitw[idx];
+    g.dist = __uint_as_float(o0.y);  // the hit buffer's distance, as the candidate pass recorded it
     g.mask = 0u, g.sel = -1;
     g.merged = (g.hitw & 0xffffffu) != 0u && a.spatial;
     if (!g.merged) return g;
     Res& res = g.res;
-    const float dist = a.cur.dist[idx];
+    const float dist = g.dist;
     uint32_t seed = a.cur.seed[idx];
     g.seed0 = seed, g.M0 = res.M;
     g.s = shading_at(a, x, y, g.hitw, dist);
@@ -452,31 +499,25 @@ __device__ __forceinline__ Merge spatial_merge(const FrameArgs& a, int x, int y)
         const int nx = x + dx, ny = y + dy;
         if ((nx == x && ny == y) || nx < 0 || ny < 0 || nx >= (int)a.W || ny >= (int)a.H) continue;
         const size_t ni = (size_t)ny * a.W + nx;
-        // the neighbour's 32-byte record: one sector instead of five planes
-        const uint4 n0 = a.nbr[2 * ni];
+        // the neighbour's record word, distance, W and invPdf (16 bytes)
+        // hold all its tests; its sample is read only once they pass
+        const uint4 n0 = a.B.r0[ni];
         if ((n0.x & 0xffffffu) == 0u) continue;
         const float nd = __uint_as_float(n0.y);
-        if (!same_face((n0.x >> 3) & (7u << 24), g.hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist))
+        if (!same_face(rec_hit_bits(n0.x), g.hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist))
             continue;
-        const uint4 n1 = a.nbr[2 * ni + 1];
-        Res nr;
-        nr.M = n0.x & 0xffffffu, nr.face = (n0.x >> 24) & 7u;
-        nr.W = __uint_as_float(n0.z), nr.invPdf = __uint_as_float(n0.w);
-        nr.pos = f3{__uint_as_float(n1.x), __uint_as_float(n1.y), __uint_as_float(n1.z)};
-        nr.light = n1.w;
-        nr.wsum = 0.0f, nr.pHat = 0.0f;  // both are recomputed by the reweigh below
+        Res nr = res_from(n0, a.B.r1[ni], make_float2(0.0f, 0.0f));  // wsum and pHat: the reweigh recomputes them
         if (nr.light >= a.L.n) continue;
         const float4 la = a.L.a[nr.light], lb = a.L.b[nr.light];
         const float ph = phat(a.point, nr.pos, face_normal(nr.face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) *
                          nr.invPdf;
         res_reweigh(nr, ph, nr.M);
         const float r0 = rnd(seed);
-        if (kMis) {
-            // UpdateReservoir's test: does the neighbour's sample take over?
-            if (r0 < nr.wsum / fmaxf(1e-5f, res.wsum + nr.wsum)) g.sel = (int)i;
-            g.mask |= 1u << i;
+        if (res_combine(res, nr, r0)) {
+            g.sel = (int)i;
+            g.shadow = 0u;
         }
-        res_combine(res, nr, r0);
+        if (kMis) g.mask |= 1u << i;
     }
     res_adjust(res, res.pHat);
     g.seed = seed;
@@ -490,7 +531,7 @@ __device__ __forceinline__ void spatial_px(const FrameArgs& a, int x, int y)
     const Merge g = spatial_merge<false>(a, x, y);
     const size_t idx = (size_t)y * a.W + x;
     if (g.merged) a.cur.seed[idx] = g.seed;
-    store_res(a.A, idx, g.res);
+    store_res(a.A, idx, g.res, g.hitw, g.dist, g.shadow);
 }
 
 // Unbiased contribution weight of the merged sample under generalized
@@ -561,11 +602,11 @@ __device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int 
             const int nx = x + dx, ny = y + dy;
             const size_t ni = (size_t)ny * a.W + nx;
             const uint32_t nh = a.cur.hitw[ni];
-            const uint4 n0 = a.nbr[2 * ni];
+            const uint4 n0 = a.B.r0[ni];
             const Shading t = shading_at(a, nx, ny, nh, __uint_as_float(n0.y));
             float p = phat(a.point, res.pos, Nl, t.sp, t.N, t.brdf, f3{lb.x, lb.y, lb.z}, la.w);
             if (p > 0.0f) {
-                const int known = tested_visibility(a.tested[ni], res.light, res.pos);
+                const int known = known_visibility(n0.x, a.B.r1[ni], res.light, res.pos);
                 if (known == 0) {
                     p = 0.0f;
                 } else if (known < 0) {
@@ -582,12 +623,13 @@ __device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int 
         }
         __syncthreads();
         const uint32_t n = *count;
+        if (tid == 0u) count_rays(a.rays, kRaysMis, n);
         for (uint32_t t = tid; t < n; t += 256u) {
             const uint32_t tk = L.task[t], ni = L.taskPx[t], owner = tk & 255u;
             const float4 smp = L.sample[owner];
             const uint32_t light = __float_as_uint(smp.w);
             const int nx = (int)(ni % a.W), ny = (int)(ni / a.W);
-            const float nd = __uint_as_float(a.nbr[2 * (size_t)ni].y);
+            const float nd = __uint_as_float(a.B.r0[ni].y);
             const Shading sh = shading_at(a, nx, ny, a.cur.hitw[ni], nd);
             const f3 spo = f3{sh.sp.x + 0.1f * sh.N.x, sh.sp.y + 0.1f * sh.N.y, sh.sp.z + 0.1f * sh.N.z};
             const f3 Ld = normalize(sub(f3{smp.x, smp.y, smp.z}, spo));
@@ -637,27 +679,43 @@ __device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
             color = f3{color.x + e.x, color.y + e.y, color.z + e.z};
         }
         if (a.gfxexp || !is_emitter(voxel)) {
-            const float4 ra = a.A.a[idx];
-            const uint32_t m = a.A.m[idx];
-            if (ra.y > 0.0f && (m & 0xffffffu) > 0u) {
-                const Res res = load_res(a.A, idx);
-                if (res.light < a.L.n) {
+            const uint4 r0 = a.A.r0[idx];
+            const float W = __uint_as_float(r0.z);
+            if (W > 0.0f && (r0.x & 0xffffffu) > 0u) {
+                const uint4 r1 = a.A.r1[idx];
+                const uint32_t light = r1.w;
+                if (light < a.L.n) {
+                    const f3 pos = f3{__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z)};
                     const f3 spo = f3{s.sp.x + 0.1f * s.N.x, s.sp.y + 0.1f * s.N.y, s.sp.z + 0.1f * s.N.z};
-                    const float4 la = a.L.a[res.light], lb = a.L.b[res.light];
-                    const f3 L = normalize(sub(res.pos, spo));
-                    const int known = tested_visibility(a.tested[idx], res.light, res.pos);
+                    const float4 la = a.L.a[light], lb = a.L.b[light];
+                    const f3 L = normalize(sub(pos, spo));
+                    // the final record's shadow bits: the candidate pass traced
+                    // this very ray (its sample survived both merges)
+                    const int known = (r0.x & kTraced) ? ((r0.x & kVisible) ? 1 : 0) : -1;
                     if (kDefer && known < 0) return true;
                     const bool visible =
                         known >= 0 ? known == 1
                                    : !light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
                     if (visible) {
-                        const f3 inc = phat_full(a.point, res.pos, face_normal(res.face), s.sp, s.N, s.brdf,
-                                                 f3{lb.x, lb.y, lb.z}, la.w);
-                        color.x += inc.x * res.W * res.invPdf;
-                        color.y += inc.y * res.W * res.invPdf;
-                        color.z += inc.z * res.W * res.invPdf;
+                        // the sample, its light and the surface again from
+                        // memory and the hit word (cache hits, the same bits),
+                        // so that little stays live through the traversal
+                        uint32_t h = hitw;
+                        size_t i2 = idx;
+                        asm volatile("" : "+v"(h), "+v"(i2));
+                        const uint4 q0 = a.A.r0[i2], q1 = a.A.r1[i2];
+                        const float4 qa = a.L.a[q1.w], qb = a.L.b[q1.w];
+                        const float Wq = __uint_as_float(q0.z), invPdf = __uint_as_float(q0.w);
+                        const f3 posq = f3{__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z)};
+                        const f3 inc = phat_full(a.point, posq, face_normal((q0.x >> 24) & 7u), s.sp, hit_normal(h),
+                                                 hit_brdf(h), f3{qb.x, qb.y, qb.z}, qa.w);
+                        color.x += inc.x * Wq * invPdf;
+                        color.y += inc.y * Wq * invPdf;
+                        color.z += inc.z * Wq * invPdf;
                     } else {
-                        a.A.a[idx] = make_float4(0.0f, 0.0f, ra.z, ra.w);
+                        // discard: W = wsum = 0 (ris.cl:276-281)
+                        reinterpret_cast<uint32_t*>(&a.A.r0[idx])[2] = 0u;
+                        reinterpret_cast<float*>(&a.A.r2[idx])[0] = 0.0f;
                     }
                 }
             }
@@ -790,7 +848,7 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
                 g.res.W = W;
                 a.cur.seed[idx] = g.seed;
             }
-            store_res(a.A, idx, g.res);
+            store_res(a.A, idx, g.res, g.hitw, g.dist, g.shadow);
         }
     } else if (in) {
         spatial_px(a, x, y);
@@ -807,6 +865,7 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
     }
     if (defer) rayPx[first + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = threadIdx.x;
     __syncthreads();
+    if (threadIdx.x == 0u) count_rays(a.rays, kRaysShading, n);
     if (threadIdx.x < n) {
         // the queued thread's pixel: same tile, its own wave quadrant and lane
         const uint32_t q = rayPx[threadIdx.x], qw = q >> 6, ql = q & 63u;

@@ -674,9 +674,9 @@ __global__ __launch_bounds__(256) void k_train_gather(FrameArgs a, const uint32_
         in[1] = (D.y * dist + a.cam.E.y) * (1.0f / 1024.0f);
         in[2] = (D.z * dist + a.cam.E.z) * (1.0f / 1024.0f);
         in[3 + mlp_face(hitw)] = 1.0f;
-        const float4 ra = a.A.a[idx];
-        const uint32_t light = __float_as_uint(a.A.b[idx].w);
-        if (ra.y > 0.0f && light < a.L.n) {
+        const float W = __uint_as_float(a.A.r0[idx].z);
+        const uint32_t light = a.A.r1[idx].w;
+        if (W > 0.0f && light < a.L.n) {
             t = clusterOf[light];
             w = 1.0f;
         }

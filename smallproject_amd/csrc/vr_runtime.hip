@@ -106,12 +106,14 @@ struct vr_ctx {
     uint32_t* hitw[2] = {};
     float* dist[2] = {};
     uint32_t* seed[2] = {};
-    float4* ra[2] = {};  // [0] = A (final), [1] = B (initial + temporal)
-    float4* rb[2] = {};
-    uint32_t* rm[2] = {};
+    // reservoir records (ResPlanes), [0] = A (final), [1] = B (initial +
+    // temporal): 16 + 16 + 8 bytes per pixel
+    uint4* rr0[2] = {};
+    uint4* rr1[2] = {};
+    float2* rr2[2] = {};
+    unsigned long long* rays = nullptr;  // [kRayKinds] traced-ray counters
+    bool countRays = false;
     float4* frame = nullptr;
-    uint4* nbr = nullptr;     // [2 per pixel] neighbour records of the B reservoirs
-    uint4* tested = nullptr;  // [per pixel] the candidate pass's shadow-ray result
     int cur = 1;              // current hit buffer (swapped before each frame)
     bool committed = false;
     bool framed = false;
@@ -137,6 +139,7 @@ struct vr_ctx {
     float* clusterPower = nullptr;      // [kMaxClusters] P_c
     float powerLow = 0.0f;              // sum of P_c over clusters with (c & 4) == 0
     float mix = 0.25f;                  // defensive mixture weight of the power source
+    uint32_t clusterDraw = VR_CLUSTER_DRAW_POWER;  // the light inside a drawn cluster
     std::vector<vr_light> hostLights;   // the uploaded light list
     uint32_t nclusters = 0, clusterLightCap = 0, clusteredLights = 0;
     uint32_t* candIdx = nullptr;  // [kCandidates][W*H]
@@ -164,6 +167,13 @@ int fail(vr_ctx* c, int code, const std::string& msg)
 {
     if (c) c->err = msg;
     return code;
+}
+
+inline float bits_float(uint32_t u)
+{
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
 }
 
 int hipfail(vr_ctx* c, hipError_t e, const char* what)
@@ -341,15 +351,12 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
         a.Nt = make_float4(p->Ntop[0], p->Ntop[1], p->Ntop[2], p->Ntop[3]);
         a.Nb = make_float4(p->Nbottom[0], p->Nbottom[1], p->Nbottom[2], p->Nbottom[3]);
     }
-    const int cur = c->cur, prev = 1 - c->cur;
+    const int cur = c->cur;
     a.cur = HitPlanes{c->hitw[cur], c->dist[cur], c->seed[cur]};
-    a.prevHitw = c->hitw[prev];
-    a.prevDist = c->dist[prev];
-    a.A = ResPlanes{c->ra[0], c->rb[0], c->rm[0]};
-    a.B = ResPlanes{c->ra[1], c->rb[1], c->rm[1]};
+    a.A = ResPlanes{c->rr0[0], c->rr1[0], c->rr2[0]};
+    a.B = ResPlanes{c->rr0[1], c->rr1[1], c->rr2[1]};
+    a.rays = c->countRays ? c->rays : nullptr;
     a.frame = c->frame;
-    a.nbr = c->nbr;
-    a.tested = c->tested;
     a.emitLut = c->emitLut;
     a.W = c->cfg.width, a.H = c->cfg.height;
     a.point = c->cfg.light_mode == VR_LIGHTS_POINT ? 1u : 0u;
@@ -500,11 +507,10 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 1024)) &&
            ok(dalloc(&c->bricks, (size_t)cfg->brick_capacity * VR_BRICKSIZE)) &&
            ok(dalloc(&c->stageIds, VR_MAXCOMMITS)) && ok(dalloc(&c->stageData, (size_t)VR_MAXCOMMITS * VR_BRICKSIZE)) &&
-           ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n)) && ok(dalloc(&c->nbr, 2 * n)) &&
-           ok(dalloc(&c->tested, n));
+           ok(dalloc(&c->emitLut, 256)) && ok(dalloc(&c->frame, n)) && ok(dalloc(&c->rays, (size_t)kRayKinds));
     for (int i = 0; good && i < 2; i++)
         good = ok(dalloc(&c->hitw[i], n)) && ok(dalloc(&c->dist[i], n)) && ok(dalloc(&c->seed[i], n)) &&
-               ok(dalloc(&c->ra[i], n)) && ok(dalloc(&c->rb[i], n)) && ok(dalloc(&c->rm[i], n));
+               ok(dalloc(&c->rr0[i], n)) && ok(dalloc(&c->rr1[i], n)) && ok(dalloc(&c->rr2[i], n));
     if (good) {
         good = ok(hipMemsetAsync(c->grid, 0, (size_t)VR_GRIDSIZE * 4, c->stream)) &&
                ok(hipMemsetAsync(c->uber, 0, 4096, c->stream)) &&
@@ -513,8 +519,9 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
                ok(hipMemsetAsync(c->frame, 0, n * 16, c->stream));
         for (int i = 0; good && i < 2; i++)
             good = ok(hipMemsetAsync(c->hitw[i], 0, n * 4, c->stream)) && ok(hipMemsetAsync(c->dist[i], 0, n * 4, c->stream)) &&
-                   ok(hipMemsetAsync(c->seed[i], 0, n * 4, c->stream)) && ok(hipMemsetAsync(c->ra[i], 0, n * 16, c->stream)) &&
-                   ok(hipMemsetAsync(c->rb[i], 0, n * 16, c->stream)) && ok(hipMemsetAsync(c->rm[i], 0, n * 4, c->stream));
+                   ok(hipMemsetAsync(c->seed[i], 0, n * 4, c->stream)) && ok(hipMemsetAsync(c->rr0[i], 0, n * 16, c->stream)) &&
+                   ok(hipMemsetAsync(c->rr1[i], 0, n * 16, c->stream)) && ok(hipMemsetAsync(c->rr2[i], 0, n * 8, c->stream));
+        good = good && ok(hipMemsetAsync(c->rays, 0, kRayKinds * 8, c->stream));
         good = good && ok(hipStreamSynchronize(c->stream));
     }
     if (!good) {
@@ -538,10 +545,10 @@ int vr_destroy(vr_ctx* c)
     if (c->side) (void)hipStreamSynchronize(c->side);
     dfree(c->grid), dfree(c->uber), dfree(c->bricks), dfree(c->stageIds), dfree(c->stageData);
     dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf), dfree(c->emitLut), dfree(c->frame);
-    dfree(c->nbr), dfree(c->tested);
+    dfree(c->rays);
     for (int i = 0; i < 2; i++) {
         dfree(c->hitw[i]), dfree(c->dist[i]), dfree(c->seed[i]);
-        dfree(c->ra[i]), dfree(c->rb[i]), dfree(c->rm[i]);
+        dfree(c->rr0[i]), dfree(c->rr1[i]), dfree(c->rr2[i]);
     }
     dfree(c->mlpP), dfree(c->mlpM), dfree(c->mlpV), dfree(c->mlpG);
     dfree(c->mlpW1b), dfree(c->mlpW2b), dfree(c->mlpW2Tb), dfree(c->mlpW3b), dfree(c->mlpW3Tb);
@@ -759,9 +766,23 @@ int vr_read_reservoirs(vr_ctx* c, int which, float* a4, float* b4, uint32_t* m)
     if (!c || which < 0 || which > 1) return VR_ERR_ARG;
     const size_t n = (size_t)c->cfg.width * c->cfg.height;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (a4) HIPCHK(c, hipMemcpy(a4, c->ra[which], n * 16, hipMemcpyDeviceToHost));
-    if (b4) HIPCHK(c, hipMemcpy(b4, c->rb[which], n * 16, hipMemcpyDeviceToHost));
-    if (m) HIPCHK(c, hipMemcpy(m, c->rm[which], n * 4, hipMemcpyDeviceToHost));
+    // the records (ResPlanes) rearranged into the documented planes
+    std::vector<uint4> r0(n), r1(n);
+    std::vector<float2> r2(n);
+    HIPCHK(c, hipMemcpy(r0.data(), c->rr0[which], n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(r1.data(), c->rr1[which], n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(r2.data(), c->rr2[which], n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        if (a4) {
+            a4[4 * i + 0] = r2[i].x, a4[4 * i + 1] = bits_float(r0[i].z);
+            a4[4 * i + 2] = r2[i].y, a4[4 * i + 3] = bits_float(r0[i].w);
+        }
+        if (b4) {
+            b4[4 * i + 0] = bits_float(r1[i].x), b4[4 * i + 1] = bits_float(r1[i].y);
+            b4[4 * i + 2] = bits_float(r1[i].z), b4[4 * i + 3] = bits_float(r1[i].w);
+        }
+        if (m) m[i] = r0[i].x & 0x07ffffffu;  // M | face << 24
+    }
     return VR_OK;
 }
 
@@ -771,6 +792,26 @@ int vr_read_frame(vr_ctx* c, float* rgba)
     const size_t n = (size_t)c->cfg.width * c->cfg.height;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(rgba, c->frame, n * 16, hipMemcpyDeviceToHost));
+    return VR_OK;
+}
+
+int vr_set_ray_counting(vr_ctx* c, int enable)
+{
+    if (!c) return VR_ERR_ARG;
+    if (enable) HIPCHK(c, hipMemsetAsync(c->rays, 0, kRayKinds * 8, c->stream));
+    c->countRays = enable != 0;
+    return VR_OK;
+}
+
+int vr_read_ray_counts(vr_ctx* c, uint64_t counts[VR_RAY_KINDS])
+{
+    if (!c || !counts) return VR_ERR_ARG;
+    static_assert(VR_RAY_KINDS == kRayKinds, "ray kinds");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->side) HIPCHK(c, hipStreamSynchronize(c->side));
+    unsigned long long v[kRayKinds];
+    HIPCHK(c, hipMemcpy(v, c->rays, sizeof v, hipMemcpyDeviceToHost));
+    for (int k = 0; k < kRayKinds; k++) counts[k] = v[k];
     return VR_OK;
 }
 
@@ -919,9 +960,9 @@ int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights
         std::vector<double> w(cnt);
         double pc = 0;
         for (uint32_t j = 0; j < cnt; j++) pc += (w[j] = pmf[lights[c0 + j]]);
-        if (!(pc > 0)) std::fill(w.begin(), w.end(), 1.0);
+        if (!(pc > 0) || c->clusterDraw == VR_CLUSTER_DRAW_UNIFORM) std::fill(w.begin(), w.end(), 1.0);
         for (uint32_t j = 0; j < cnt; j++)
-            share[lights[c0 + j]] = (float)(pc > 0 ? w[j] / pc : 1.0 / cnt);
+            share[lights[c0 + j]] = (float)(pc > 0 && c->clusterDraw == VR_CLUSTER_DRAW_POWER ? w[j] / pc : 1.0 / cnt);
         vose(w.data(), cnt, prob.data() + c0, alias.data() + c0);
         power[k] = (float)pc;
         if ((k & 4u) == 0u) low += pc;
@@ -952,6 +993,15 @@ int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights
     c->powerLow = (float)low;
     c->nclusters = nclusters;
     c->clusteredLights = nlights;
+    return VR_OK;
+}
+
+int vr_set_cluster_draw(vr_ctx* c, uint32_t mode)
+{
+    if (!c) return VR_ERR_ARG;
+    if (mode > VR_CLUSTER_DRAW_UNIFORM) return fail(c, VR_ERR_ARG, "vr_set_cluster_draw: unknown mode");
+    c->clusterDraw = mode;
+    c->clusteredLights = 0;  // the tables follow at the next vr_set_light_clusters
     return VR_OK;
 }
 

@@ -18,10 +18,11 @@ from reset temporal history; per-frame GPU time comes from the stage
 events (the learned method's proposal kernel is part of its candidate
 stage).  Before the comparison the network is trained online along the
 configuration's camera path from the plain method's final reservoirs (2^16
-samples per frame in 16 minibatches, Adam lr 1e-3); during the comparison
-its weights are frozen.  Reported: the plain method's MSE after 8 frames,
-the number of learned frames (and their time) needed to reach it, and the
-learned MSE at the plain method's 8-frame time.
+samples per frame in 16 minibatches, Adam lr 1e-3).  During the comparison
+it keeps training after every frame on its own frame's reservoirs, and that
+training's time is added to its frame time.  Reported: the plain method's
+MSE after 8 frames, the number of learned frames (and their time) needed to
+reach it, and the learned MSE at the plain method's 8-frame time.
 """
 from __future__ import annotations
 
@@ -72,19 +73,32 @@ def mse(gt_rgb: np.ndarray, frame_rgba: np.ndarray, width: int, height: int) -> 
     return image_error(gt, frame_rgba, width, height)[0]
 
 
-def render_sequence(world, renderer, pos, target, frames: int, gt_rgb, width: int, height: int):
+def render_sequence(world, renderer, pos, target, frames: int, gt_rgb, width: int, height: int, train=None,
+                    keep=None):
     """`frames` frames at a fixed camera from reset temporal history: MSE
-    after each frame and GPU ms per frame."""
+    after each frame and ms per frame: the frame's GPU stage time, plus the
+    wall time of `train()` after each frame when given (online training
+    charged to the frame).  keep(f, renderer) sees each rendered frame."""
     world.look_at(pos, target)
     world.params.restirtemporalframe = 0
     renderer.sync()
     renderer.profile_begin()
     errs = []
-    for _ in range(frames):
+    train_ms = 0.0
+    for f in range(frames):
         world.render()
         errs.append(mse(gt_rgb, renderer.frame(), width, height))
+        if keep is not None:
+            keep(f, renderer)
+        if train is not None:
+            renderer.sync()
+            t0 = time.perf_counter()
+            train()
+            renderer.sync()
+            train_ms += 1000.0 * (time.perf_counter() - t0)
     s, n = renderer.profile_read()
-    return errs, float(s.sum()) / max(n, 1), [float(v) / max(n, 1) for v in s]
+    stage = [float(v) / max(n, 1) for v in s]
+    return errs, float(s.sum()) / max(n, 1) + train_ms / max(frames, 1), stage + [train_ms / max(frames, 1)]
 
 
 def plain_mse(name: str, width=None, height=None, frames: int = 8, gt_spp: int = 4096, eval_frame=None,
@@ -110,11 +124,21 @@ def plain_mse(name: str, width=None, height=None, frames: int = 8, gt_spp: int =
 
 def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_frames=None, eval_frames: int = 16,
                gt_spp: int = 4096, nsamples: int = 1 << 16, lr: float = 1e-3, train_steps: int = 16,
-               nclusters: int = 256, eval_frame=None, mix: float = 0.25, flags: int = 0) -> dict:
+               nclusters: int = 256, eval_frame=None, mix: float = 0.25, flags: int = 0,
+               draw: int = _lib.CLUSTER_DRAW_POWER, online: bool = True, gt_cache: dict | None = None,
+               keep=None) -> dict:
     """Learned proposal vs the plain power source on one camera of the
-    configuration's path: online training from the plain method's frames,
-    then MSE per frame of both against the converged reference.  `flags`
-    (vr_config.flags) applies to both methods."""
+    configuration's path: online training from the plain method's frames
+    along the path, then MSE per frame of both against the converged
+    reference.  With `online` the proposal keeps training during the
+    comparison, after every frame on that frame's own reservoirs (nsamples,
+    train_steps Adam steps), and the training's time is charged to its
+    frames.  `flags` (vr_config.flags) applies to both methods; `mix` is the
+    defensive mixture weight and `draw` the in-cluster draw of the proposal
+    (eps = 0 with CLUSTER_DRAW_UNIFORM is the many-light configuration as
+    stated: softmax over clusters, then uniform within the cluster).
+    `gt_cache` shares converged references between calls;
+    keep(f, renderer) sees each learned frame."""
     base = CONFIGS[name]
     scP = dataclasses.replace(base, source=_lib.SOURCE_POWER, quirks=EVAL_QUIRKS, flags=flags)
     scL = dataclasses.replace(base, source=_lib.SOURCE_LEARNED, quirks=EVAL_QUIRKS, flags=flags)
@@ -126,6 +150,7 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
     wL.force_sync()
     rL = wL.renderer
     clusters = wL.cluster_lights(min(nclusters, len(wL.lights())), 8, 1)
+    rL.set_cluster_draw(draw)
     rL.set_light_clusters(clusters, int(clusters.max()) + 1)
     rL.set_proposal_mix(mix)
     # online training along the camera path from the plain method's frames
@@ -142,11 +167,19 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
          f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
     f = base.frames // 2 if eval_frame is None else eval_frame
     pos, tgt = base.camera(f)
-    t0 = time.perf_counter()
-    gt, noise = converged_reference(rP, camera_params(wP, pos, tgt), gt_spp)
-    _log(f"evaluate: converged reference ({gt_spp} spp) in {time.perf_counter() - t0:.1f} s, noise {noise:.3g}")
+    key = (name, W, H, f, gt_spp)
+    if gt_cache is not None and key in gt_cache:
+        gt, noise = gt_cache[key]
+        camera_params(wP, pos, tgt)
+    else:
+        t0 = time.perf_counter()
+        gt, noise = converged_reference(rP, camera_params(wP, pos, tgt), gt_spp)
+        _log(f"evaluate: converged reference ({gt_spp} spp) in {time.perf_counter() - t0:.1f} s, noise {noise:.3g}")
+        if gt_cache is not None:
+            gt_cache[key] = (gt, noise)
     eP, msP, stP = render_sequence(wP, rP, pos, tgt, eval_frames, gt, W, H)
-    eL, msL, stL = render_sequence(wL, rL, pos, tgt, eval_frames, gt, W, H)
+    train = (lambda: rL.mlp_train(nsamples, lr, train_steps)) if online else None
+    eL, msL, stL = render_sequence(wL, rL, pos, tgt, eval_frames, gt, W, H, train=train, keep=keep)
     k_ref = min(8, eval_frames)
     target = eP[k_ref - 1]
     reach = next((k + 1 for k, e in enumerate(eL) if e <= target), None)
@@ -154,7 +187,9 @@ def equal_time(name: str = "c3_clusters_1024", width=None, height=None, train_fr
     out = {
         "config": name, "resolution": [W, H], "camera_frame": f, "gt_spp": gt_spp, "gt_noise_mse": noise,
         "spatial_merge": "mis" if flags & _lib.FLAG_MIS_SPATIAL else "biased",
-        "clusters": int(clusters.max()) + 1, "mix": mix, "train_frames": T, "train_steps_per_frame": train_steps,
+        "clusters": int(clusters.max()) + 1, "mix": mix,
+        "in_cluster_draw": "uniform" if draw == _lib.CLUSTER_DRAW_UNIFORM else "power",
+        "online_during_comparison": online, "train_frames": T, "train_steps_per_frame": train_steps,
         "train_samples": nsamples, "lr": lr, "train_loss_first": losses[0], "train_loss_last": losses[-1],
         "plain": {"ms_per_frame": msP, "stage_ms": stP, "mse_per_frame": eP},
         "learned": {"ms_per_frame": msL, "stage_ms": stL, "mse_per_frame": eL},

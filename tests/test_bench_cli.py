@@ -22,5 +22,7 @@ def test_stage_bytes_cover_every_stage():
     from smallproject_amd import _lib
     assert set(bench.STAGE_BYTES) == set(_lib.STAGE_NAMES)
     # SoA planes: hit 12 B; reservoir 36 B; seed 4 B; frame 16 B (per pixel)
-    assert bench.STAGE_BYTES == {"primary": 12, "initial": 12 + 8 + 36 + 36 + 4, "resolve": 36 + 12 + 36 + 4 + 16}
+    # records: 16-B test word + 16-B sample + 8-B sums; the temporal pass
+    # reads the word and the sample of the previous frame's record
+    assert bench.STAGE_BYTES == {"primary": 12, "initial": 12 + 16 + 16 + 40 + 4, "resolve": 40 + 12 + 40 + 4 + 16}
     assert bench.METRIC.startswith("1080p fps")

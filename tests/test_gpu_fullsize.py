@@ -93,3 +93,13 @@ def test_c5_moving_scene_24_frames_bit_exact(gpu):
     for f, cmp in enumerate(res):
         assert cmp.ok, f"frame {f}\n{cmp.report()}"
 
+
+
+def test_letters_scene_bit_exact(gpu):
+    """The reference's own letters scene file (scene_export/letters/grid.vx,
+    1249 emissive voxels) at the reference's 1600x1024, from its recorded
+    first camera angle with focal distance 2.2, area lights, M=32: three
+    frames (temporal reuse from the second on) bit-exact."""
+    sc = configs.CONFIGS["letters_1600"]
+    res = run_config(sc, frames=3)
+    _check(res, "letters")
