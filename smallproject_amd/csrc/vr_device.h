@@ -30,26 +30,32 @@ struct WorldView {
 struct LightView {
     const float4* __restrict__ a;
     const float4* __restrict__ b;
-    const float* __restrict__ prob;     // alias table (power source) or null
-    const uint32_t* __restrict__ alias;
-    const float* __restrict__ pmf;
+    // power source: bucket i of the alias table as (prob_i, alias_i bits,
+    // pmf_i, pmf_alias_i), so one 16-byte load yields the light and its pdf;
+    // null for the uniform source
+    const float4* __restrict__ pick;
     uint32_t n;
 };
 
 // ---------------------------------------------------------------------------
 // scalar helpers
 // ---------------------------------------------------------------------------
+// Saturating conversions, truncating toward zero, NaN -> 0.  gfx950's
+// v_cvt_u32_f32 / v_cvt_i32_f32 define exactly this for every input, so each
+// is one instruction; written in C++ the range checks cost 4 selects (f2u)
+// or a branch nest (f2i) per conversion.
 __device__ __forceinline__ uint32_t f2u(float x)
 {
-    return x > 0.0f ? (x >= 4294967296.0f ? 0xffffffffu : (uint32_t)x) : 0u;
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 __device__ __forceinline__ int32_t f2i(float x)
 {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return 2147483647;
-    if (x <= -2147483648.0f) return -2147483647 - 1;
-    return (int32_t)x;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 __device__ __forceinline__ uint32_t clampu(uint32_t v, uint32_t lo, uint32_t hi)

@@ -56,11 +56,13 @@ struct FrameArgs {
     // with the cluster's power alias table
     const uint32_t* candIdx;
     const float* candPdf;
-    const uint32_t* clusterStart;  // [257] offsets into clusterLights
-    const uint32_t* clusterLights; // light ids grouped by cluster
-    const float* clusterProb;      // alias table per cluster, aligned with clusterLights
-    const uint32_t* clusterAlias;  // (cluster-local indices)
-    const float* lightShare;       // [lights] pmf_i / P_c of the light's cluster
+    const uint32_t* clusterStart;  // [257] offsets into the cluster-ordered light list
+    // bucket j of the list (cluster c's buckets are clusterStart[c]..): the
+    // alias table's (prob bits, light j, light of alias j, alias j local
+    // index) and the two lights' in-cluster shares pmf_i / P_c, so one pair
+    // of loads yields the light and its pdf
+    const uint4* clusterPick;
+    const float2* clusterPickShare;
 };
 
 // Tile order (block id -> 16x16 tile).  Workgroups are dealt round-robin to

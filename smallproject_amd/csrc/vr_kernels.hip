@@ -231,9 +231,10 @@ __device__ __forceinline__ uint32_t pick_light(const LightView& L, uint32_t sour
     i = i < 0 ? 0 : (i > (int32_t)N - 1 ? (int32_t)N - 1 : i);
     if (source == 1u) {
         const float fr = u * (float)N - (float)i;
-        const uint32_t k = fr < L.prob[i] ? (uint32_t)i : L.alias[i];
-        p = L.pmf[k];
-        return k;
+        const float4 e = L.pick[i];
+        const bool keep = fr < e.x;
+        p = keep ? e.z : e.w;  // pmf of the light returned
+        return keep ? (uint32_t)i : __float_as_uint(e.y);
     }
     p = 1.0f / (float)N;
     return (uint32_t)i;
@@ -364,9 +365,11 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
                 int32_t k = f2i(u * (float)cnt);
                 k = k < 0 ? 0 : (k > (int32_t)cnt - 1 ? (int32_t)cnt - 1 : k);
                 const float fr = u * (float)cnt - (float)k;
-                const uint32_t kk = fr < a.clusterProb[c0 + k] ? (uint32_t)k : a.clusterAlias[c0 + k];
-                li = a.clusterLights[c0 + kk];
-                pr = mc * a.lightShare[li];
+                const uint4 e = a.clusterPick[c0 + k];
+                const float2 sh = a.clusterPickShare[c0 + k];
+                const bool keep = fr < __uint_as_float(e.x);
+                li = keep ? e.y : e.z;
+                pr = mc * (keep ? sh.x : sh.y);
             } else {
                 li = pick_light(a.L, a.source, seed, pr);
             }

@@ -97,9 +97,7 @@ struct vr_ctx {
     // lights
     float4* la = nullptr;
     float4* lb = nullptr;
-    float* prob = nullptr;
-    uint32_t* alias = nullptr;
-    float* pmf = nullptr;
+    float4* pick = nullptr;  // power source's alias table (LightView::pick)
     uint32_t nLights = 0, lightCap = 0;
     float* emitLut = nullptr;
     // per-pixel planes
@@ -131,11 +129,9 @@ struct vr_ctx {
     __bf16* mlpW3Tb = nullptr;
     uint32_t adamStep = 0;
     uint32_t* clusterStart = nullptr;   // [kMaxClusters + 1]
-    uint32_t* clusterLights = nullptr;  // [lights] light ids grouped by cluster
     uint32_t* clusterOf = nullptr;      // [lights]
-    float* clusterProb = nullptr;       // [lights] per-cluster power alias tables
-    uint32_t* clusterAlias = nullptr;   // [lights] (cluster-local indices)
-    float* lightShare = nullptr;        // [lights] pmf_i / P_c
+    uint4* clusterPick = nullptr;       // [lights] FrameArgs::clusterPick
+    float2* clusterPickShare = nullptr; // [lights] FrameArgs::clusterPickShare
     float* clusterPower = nullptr;      // [kMaxClusters] P_c
     float powerLow = 0.0f;              // sum of P_c over clusters with (c & 4) == 0
     float mix = 0.25f;                  // defensive mixture weight of the power source
@@ -331,7 +327,7 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
     FrameArgs a;
     std::memset(&a, 0, sizeof(a));
     a.w = WorldView{c->grid, c->bricks, c->uber};
-    a.L = LightView{c->la, c->lb, c->prob, c->alias, c->pmf, p ? p->numberOfLights : 0u};
+    a.L = LightView{c->la, c->lb, c->pick, p ? p->numberOfLights : 0u};
     if (p) {
         a.cam.ox = p->oneOverRes[0], a.cam.oy = p->oneOverRes[1];
         a.cam.E = f3{p->E[0], p->E[1], p->E[2]};
@@ -370,10 +366,8 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
     a.candIdx = c->candIdx;
     a.candPdf = c->candPdf;
     a.clusterStart = c->clusterStart;
-    a.clusterLights = c->clusterLights;
-    a.clusterProb = c->clusterProb;
-    a.clusterAlias = c->clusterAlias;
-    a.lightShare = c->lightShare;
+    a.clusterPick = c->clusterPick;
+    a.clusterPickShare = c->clusterPickShare;
     return a;
 }
 
@@ -544,7 +538,7 @@ int vr_destroy(vr_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->side) (void)hipStreamSynchronize(c->side);
     dfree(c->grid), dfree(c->uber), dfree(c->bricks), dfree(c->stageIds), dfree(c->stageData);
-    dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf), dfree(c->emitLut), dfree(c->frame);
+    dfree(c->la), dfree(c->lb), dfree(c->pick), dfree(c->emitLut), dfree(c->frame);
     dfree(c->rays);
     for (int i = 0; i < 2; i++) {
         dfree(c->hitw[i]), dfree(c->dist[i]), dfree(c->seed[i]);
@@ -552,8 +546,8 @@ int vr_destroy(vr_ctx* c)
     }
     dfree(c->mlpP), dfree(c->mlpM), dfree(c->mlpV), dfree(c->mlpG);
     dfree(c->mlpW1b), dfree(c->mlpW2b), dfree(c->mlpW2Tb), dfree(c->mlpW3b), dfree(c->mlpW3Tb);
-    dfree(c->clusterStart), dfree(c->clusterLights), dfree(c->clusterOf), dfree(c->clusterProb);
-    dfree(c->clusterAlias), dfree(c->lightShare), dfree(c->clusterPower);
+    dfree(c->clusterStart), dfree(c->clusterOf), dfree(c->clusterPick), dfree(c->clusterPickShare);
+    dfree(c->clusterPower);
     dfree(c->candIdx), dfree(c->candPdf);
     dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
     dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget), dfree(c->claim);
@@ -612,11 +606,11 @@ int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (n > c->lightCap) {
-        dfree(c->la), dfree(c->lb), dfree(c->prob), dfree(c->alias), dfree(c->pmf);
-        c->la = c->lb = nullptr, c->prob = c->pmf = nullptr, c->alias = nullptr;
+        dfree(c->la), dfree(c->lb), dfree(c->pick);
+        c->la = c->lb = c->pick = nullptr;
         const uint32_t cap = n * 2;
-        if (dalloc(&c->la, cap) != hipSuccess || dalloc(&c->lb, cap) != hipSuccess || dalloc(&c->prob, cap) != hipSuccess ||
-            dalloc(&c->alias, cap) != hipSuccess || dalloc(&c->pmf, cap) != hipSuccess) {
+        if (dalloc(&c->la, cap) != hipSuccess || dalloc(&c->lb, cap) != hipSuccess ||
+            dalloc(&c->pick, cap) != hipSuccess) {
             // the old list's buffers are gone: leave an empty list, so that
             // no later frame or step can reach the freed records
             c->lightCap = 0, c->nLights = 0, c->motionN = 0;
@@ -637,9 +631,14 @@ int vr_set_lights(vr_ctx* c, const vr_light* lights, uint32_t n)
         std::vector<float> prob, pmf;
         std::vector<uint32_t> alias;
         power_table(c->cfg, lights, n, prob, alias, pmf);
-        HIPCHK(c, hipMemcpyAsync(c->prob, prob.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->alias, alias.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->pmf, pmf.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        std::vector<float4> pick(n);
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t a = alias[i];
+            float ab;
+            std::memcpy(&ab, &a, 4);
+            pick[i] = make_float4(prob[i], ab, pmf[i], pmf[a]);
+        }
+        HIPCHK(c, hipMemcpyAsync(c->pick, pick.data(), (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return VR_OK;
@@ -968,26 +967,37 @@ int vr_set_light_clusters(vr_ctx* c, const uint32_t* clusterOf, uint32_t nlights
         if ((k & 4u) == 0u) low += pc;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // bucket j of cluster k (global index c0 + j): prob, its own light, its
+    // alias's light and local index, and both lights' shares
+    std::vector<uint4> pick(nlights);
+    std::vector<float2> pickShare(nlights);
+    for (uint32_t k = 0; k < (uint32_t)kMaxClusters; k++) {
+        const uint32_t c0 = start[k], cnt = start[k + 1] - c0;
+        for (uint32_t j = 0; j < cnt; j++) {
+            const uint32_t self = lights[c0 + j], other = lights[c0 + alias[c0 + j]];
+            uint32_t pb;
+            std::memcpy(&pb, &prob[c0 + j], 4);
+            pick[c0 + j] = make_uint4(pb, self, other, alias[c0 + j]);
+            pickShare[c0 + j] = make_float2(share[self], share[other]);
+        }
+    }
     if (nlights > c->clusterLightCap) {
-        dfree(c->clusterLights), dfree(c->clusterOf), dfree(c->clusterProb), dfree(c->clusterAlias), dfree(c->lightShare);
-        c->clusterLights = nullptr, c->clusterOf = nullptr, c->clusterProb = nullptr, c->clusterAlias = nullptr,
-        c->lightShare = nullptr, c->clusterLightCap = 0;
+        dfree(c->clusterOf), dfree(c->clusterPick), dfree(c->clusterPickShare);
+        c->clusterOf = nullptr, c->clusterPick = nullptr, c->clusterPickShare = nullptr, c->clusterLightCap = 0;
         // the old tables are gone: no frame may use them if this fails
         c->nclusters = 0, c->clusteredLights = 0;
-        if (dalloc(&c->clusterLights, nlights) != hipSuccess || dalloc(&c->clusterOf, nlights) != hipSuccess ||
-            dalloc(&c->clusterProb, nlights) != hipSuccess || dalloc(&c->clusterAlias, nlights) != hipSuccess ||
-            dalloc(&c->lightShare, nlights) != hipSuccess)
+        if (dalloc(&c->clusterOf, nlights) != hipSuccess || dalloc(&c->clusterPick, nlights) != hipSuccess ||
+            dalloc(&c->clusterPickShare, nlights) != hipSuccess)
             return fail(c, VR_ERR_NOMEM, "vr_set_light_clusters: allocation failed");
         c->clusterLightCap = nlights;
     }
     HIPCHK(c, hipMemcpyAsync(c->clusterStart, start.data(), (kMaxClusters + 1) * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->clusterPower, power.data(), kMaxClusters * 4, hipMemcpyHostToDevice, c->stream));
     if (nlights) {
-        HIPCHK(c, hipMemcpyAsync(c->clusterLights, lights.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->clusterOf, clusterOf, (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->clusterProb, prob.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->clusterAlias, alias.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->lightShare, share.data(), (size_t)nlights * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->clusterPick, pick.data(), (size_t)nlights * 16, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->clusterPickShare, pickShare.data(), (size_t)nlights * 8, hipMemcpyHostToDevice,
+                                 c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->powerLow = (float)low;
@@ -1022,10 +1032,18 @@ int vr_read_cluster_tables(vr_ctx* c, uint32_t* start, uint32_t* lights, float* 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (start) HIPCHK(c, hipMemcpy(start, c->clusterStart, (kMaxClusters + 1) * 4, hipMemcpyDeviceToHost));
     if (power) HIPCHK(c, hipMemcpy(power, c->clusterPower, kMaxClusters * 4, hipMemcpyDeviceToHost));
-    if (n && lights) HIPCHK(c, hipMemcpy(lights, c->clusterLights, n * 4, hipMemcpyDeviceToHost));
-    if (n && prob) HIPCHK(c, hipMemcpy(prob, c->clusterProb, n * 4, hipMemcpyDeviceToHost));
-    if (n && alias) HIPCHK(c, hipMemcpy(alias, c->clusterAlias, n * 4, hipMemcpyDeviceToHost));
-    if (n && share) HIPCHK(c, hipMemcpy(share, c->lightShare, n * 4, hipMemcpyDeviceToHost));
+    if (n == 0) return VR_OK;
+    // the tables as the kernels read them, unpacked
+    std::vector<uint4> pick(n);
+    std::vector<float2> pickShare(n);
+    HIPCHK(c, hipMemcpy(pick.data(), c->clusterPick, n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(pickShare.data(), c->clusterPickShare, n * 8, hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < n; j++) {
+        if (lights) lights[j] = pick[j].y;
+        if (prob) std::memcpy(&prob[j], &pick[j].x, 4);
+        if (alias) alias[j] = pick[j].w;
+        if (share && pick[j].y < n) share[pick[j].y] = pickShare[j].x;
+    }
     return VR_OK;
 }
 
