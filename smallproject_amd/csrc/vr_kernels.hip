@@ -328,6 +328,26 @@ __device__ __forceinline__ Shading shading_at(const FrameArgs& a, int x, int y, 
 // ---------------------------------------------------------------------------
 // k_initial: candidates + visibility + temporal reuse (cl/ris.cl:20-124)
 // ---------------------------------------------------------------------------
+// LDS of the candidate pass (256 threads per workgroup, one slot each,
+// planes of 16 bytes so that neighbouring threads hit different banks):
+// what stays live through the shadow ray's traversal
+struct ParkLds {
+    float4 a[256], b[256], c[256];
+    uint2 m[256];
+};
+
+__device__ __forceinline__ ParkLds& park_lds()
+{
+    __shared__ ParkLds park;
+    return park;
+}
+
+__device__ __forceinline__ float4 vload(const float4& p)
+{
+    const volatile float4& v = p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
 {
     const size_t idx = (size_t)y * a.W + x;
@@ -386,25 +406,42 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
             res_update(res, ph / pr, ph, li, rnd(seed), pos, invPdf, face);
         }
         res_adjust(res, res.pHat);
+        // The reservoir, the shading point and the stream wait in LDS while
+        // the shadow ray traverses the grid, and are read back after it: the
+        // traversal then has the registers to itself instead of spilling
+        // them to scratch memory (HBM traffic).  The reads are volatile, so
+        // they are not replaced by the values stored.
+        ParkLds& pk = park_lds();
+        const uint32_t t = threadIdx.x;
+        pk.a[t] = make_float4(res.wsum, res.W, res.pHat, res.invPdf);
+        pk.b[t] = make_float4(res.pos.x, res.pos.y, res.pos.z, __uint_as_float(res.light));
+        pk.c[t] = make_float4(s.sp.x, s.sp.y, s.sp.z, __uint_as_float(seed));
+        pk.m[t] = make_uint2(res.M, res.face);
+        bool occ = false;
         if (res.wsum != 0.0f) {
             // with wsum == 0 the discard would leave W = wsum = 0 unchanged
             const float4 lb = a.L.b[res.light];
             const float4 la = a.L.a[res.light];
             const f3 L = normalize(sub(res.pos, spo));
-            const bool occ = light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
-            if (occ) res.W = 0.0f, res.wsum = 0.0f;
+            occ = light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
             shadow = kTraced | (occ ? 0u : kVisible);
             traced = 1u;
         }
+        const float4 pa = vload(pk.a[t]), pb = vload(pk.b[t]), pc = vload(pk.c[t]);
+        const volatile uint2& pm = pk.m[t];
+        res.wsum = occ ? 0.0f : pa.x, res.W = occ ? 0.0f : pa.y, res.pHat = pa.z, res.invPdf = pa.w;
+        res.pos = f3{pb.x, pb.y, pb.z}, res.light = __float_as_uint(pb.w);
+        res.M = pm.x, res.face = pm.y;
+        seed = __float_as_uint(pc.w);
         if (a.restirtemporalframe > 0u && a.temporal) {
             // The normal and brdf again from the hit word (the same bits:
             // the word carries the side and the sign of D[side], and the
-            // colour): nothing but the shading point stays live through the
-            // shadow ray's traversal.  The empty asm keeps the compiler from
-            // reusing the values computed before it.
+            // colour), the shading point from LDS: nothing of the shading
+            // stays live through the shadow ray's traversal.  The empty asm
+            // keeps the compiler from reusing the values computed before it.
             uint32_t hw = hitw;
             asm volatile("" : "+v"(hw));
-            const Shading s2 = Shading{s.D, hit_normal(hw), hit_brdf(hw), s.sp};
+            const Shading s2 = Shading{s.D, hit_normal(hw), hit_brdf(hw), f3{pc.x, pc.y, pc.z}};
             const Shading& s = s2;
             // ReprojectWorldPoint (cl/tools.cl:479-488)
             const float dl = dot(s.sp, f3{a.Nl.x, a.Nl.y, a.Nl.z}) - a.Nl.w;
@@ -664,14 +701,33 @@ __device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int 
 // did not trace is left alone (no frame or reservoir write) and true is
 // returned; shading it again with kDefer = false gives the one-pass result,
 // because nothing it reads has changed.
-template <bool kDefer = false>
+// LDS of the shading pass (one slot per thread): the partial colour, the
+// distance, the shading point and the pixel index wait here while the
+// shadow ray traverses the grid, as the candidate pass parks its reservoir
+struct ShadeParkLds {
+    float4 c[256], p[256];
+};
+
+__device__ __forceinline__ ShadeParkLds& shade_park_lds()
+{
+    __shared__ ShadeParkLds park;
+    return park;
+}
+
+// kPark: the call that traces (kDefer false) parks the live values in LDS
+// across the traversal (the MIS merge's kernel has no LDS to spare for it)
+template <bool kDefer = false, bool kPark = false>
 __device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
 {
+    constexpr bool park = kPark && !kDefer;
     const size_t idx = (size_t)y * a.W + x;
     const uint32_t hitw = a.cur.hitw[idx];
     const uint32_t voxel = hitw & 0xffffffu;
     const float dist = a.cur.dist[idx];
     f3 color = f3{0.0f, 0.0f, 0.0f};
+    size_t oidx = idx;
+    float odist = dist;
+    uint32_t ovoxel = voxel;
     if (voxel != 0u) {
         const Shading s = shading_at(a, x, y, hitw, dist);
         if (is_emitter(voxel)) {
@@ -681,6 +737,13 @@ __device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
             if (a.gfxexp) e = f3{e.x * kInvPi, e.y * kInvPi, e.z * kInvPi};
             color = f3{color.x + e.x, color.y + e.y, color.z + e.z};
         }
+        ShadeParkLds& pk = shade_park_lds();
+        const uint32_t t = threadIdx.x;
+        if (park) {
+            pk.c[t] = make_float4(color.x, color.y, color.z, dist);
+            pk.p[t] = make_float4(s.sp.x, s.sp.y, s.sp.z, __uint_as_float((uint32_t)idx));
+        }
+        f3 add = f3{0.0f, 0.0f, 0.0f};
         if (a.gfxexp || !is_emitter(voxel)) {
             const uint4 r0 = a.A.r0[idx];
             const float W = __uint_as_float(r0.z);
@@ -699,38 +762,50 @@ __device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
                     const bool visible =
                         known >= 0 ? known == 1
                                    : !light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
+                    // what the traversal did not keep in registers: from LDS
+                    // when parked, else the values themselves
+                    const float4 pp = park ? vload(pk.p[t]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    const size_t pidx = park ? (size_t)__float_as_uint(pp.w) : idx;
                     if (visible) {
                         // the sample, its light and the surface again from
                         // memory and the hit word (cache hits, the same bits),
                         // so that little stays live through the traversal
-                        uint32_t h = hitw;
-                        size_t i2 = idx;
+                        uint32_t h = park ? 0u : hitw;
+                        size_t i2 = pidx;
                         asm volatile("" : "+v"(h), "+v"(i2));
+                        if (park) h = a.cur.hitw[i2];
+                        const f3 sp = park ? f3{pp.x, pp.y, pp.z} : s.sp;
                         const uint4 q0 = a.A.r0[i2], q1 = a.A.r1[i2];
                         const float4 qa = a.L.a[q1.w], qb = a.L.b[q1.w];
                         const float Wq = __uint_as_float(q0.z), invPdf = __uint_as_float(q0.w);
                         const f3 posq = f3{__uint_as_float(q1.x), __uint_as_float(q1.y), __uint_as_float(q1.z)};
-                        const f3 inc = phat_full(a.point, posq, face_normal((q0.x >> 24) & 7u), s.sp, hit_normal(h),
+                        const f3 inc = phat_full(a.point, posq, face_normal((q0.x >> 24) & 7u), sp, hit_normal(h),
                                                  hit_brdf(h), f3{qb.x, qb.y, qb.z}, qa.w);
-                        color.x += inc.x * Wq * invPdf;
-                        color.y += inc.y * Wq * invPdf;
-                        color.z += inc.z * Wq * invPdf;
+                        add = f3{inc.x * Wq * invPdf, inc.y * Wq * invPdf, inc.z * Wq * invPdf};
                     } else {
                         // discard: W = wsum = 0 (ris.cl:276-281)
-                        reinterpret_cast<uint32_t*>(&a.A.r0[idx])[2] = 0u;
-                        reinterpret_cast<float*>(&a.A.r2[idx])[0] = 0.0f;
+                        reinterpret_cast<uint32_t*>(&a.A.r0[pidx])[2] = 0u;
+                        reinterpret_cast<float*>(&a.A.r2[pidx])[0] = 0.0f;
                     }
                 }
             }
         }
+        if (park) {
+            const float4 pc = vload(pk.c[t]);
+            const float4 pp = vload(pk.p[t]);
+            color = f3{pc.x, pc.y, pc.z}, odist = pc.w, oidx = (size_t)__float_as_uint(pp.w);
+            ovoxel = a.cur.hitw[oidx] & 0xffffffu;
+        }
+        // color + 0 = color exactly (color is never -0)
+        color = f3{color.x + add.x, color.y + add.y, color.z + add.z};
         if (a.doubleRadiance) {
-            const float al = alphaf(voxel);
+            const float al = alphaf(ovoxel);
             color.x += al * color.x + (1 - al) * 0.0f;
             color.y += al * color.y + (1 - al) * 0.0f;
             color.z += al * color.z + (1 - al) * 0.0f;
         }
     }
-    a.frame[idx] = make_float4(color.x, color.y, color.z, dist);
+    a.frame[oidx] = make_float4(color.x, color.y, color.z, odist);
     return false;
 }
 
@@ -835,7 +910,9 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
 // order) and shaded by the first threads, so they fill as few waves as
 // possible and the other waves finish at once.  Uniform control flow: every
 // thread reaches every barrier (the MIS weight's included).
-template <bool kMis>
+// kPark: shading parks its live values in LDS across the traversal (not
+// where a kernel also holds the MIS merge's LDS)
+template <bool kMis, bool kPark = !kMis>
 __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
 {
     __shared__ uint32_t rayPx[256];  // threads of the tile whose pixel needs a ray
@@ -856,7 +933,7 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
     } else if (in) {
         spatial_px(a, x, y);
     }
-    const bool defer = in && shade_px<true>(a, x, y);
+    const bool defer = in && shade_px<true, kPark>(a, x, y);
     const uint64_t m = __ballot(defer);
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     if (lane == 0u) waveRays[wv] = (uint32_t)__popcll(m);
@@ -873,7 +950,7 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
         // the queued thread's pixel: same tile, its own wave quadrant and lane
         const uint32_t q = rayPx[threadIdx.x], qw = q >> 6, ql = q & 63u;
         const int x0 = x - (int)((wv & 1u) * 8u + (lane & 7u)), y0 = y - (int)((wv >> 1) * 8u + (lane >> 3));
-        shade_px(a, x0 + (int)((qw & 1u) * 8u + (ql & 7u)), y0 + (int)((qw >> 1) * 8u + (ql >> 3)));
+        shade_px<false, kPark>(a, x0 + (int)((qw & 1u) * 8u + (ql & 7u)), y0 + (int)((qw >> 1) * 8u + (ql >> 3)));
     }
 }
 
@@ -906,7 +983,7 @@ __global__ __launch_bounds__(256) VR_WAVES(kAnyMis ? kDefaultWavesMis : 7) void 
 {
     many_views<kPersistent, true>(c, [](const FrameArgs& v, int x, int y) {
         if (kAnyMis && v.misSpatial) resolve_tile<true>(v, x, y);
-        else resolve_tile<false>(v, x, y);
+        else resolve_tile<false, !kAnyMis>(v, x, y);
     });
 }
 

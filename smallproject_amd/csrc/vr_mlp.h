@@ -69,7 +69,12 @@ hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint32_t* candI
 hipError_t launch_mlp_probs(const MlpView& m, const float* x, uint32_t B, float* out, hipStream_t s);
 hipError_t launch_mlp_grad(const MlpView& m, const float* x, const uint32_t* t, const float* w, uint32_t B,
                            float* partial, float* loss, float* grad, hipStream_t s);
-hipError_t launch_adam(float* P, const float* g, float* mo, float* ve, float lr, uint32_t t, hipStream_t s);
+// one Adam step on B samples: gradient chunks, then their sum, the update
+// and the BF16 copies in one kernel (the gradient bits of launch_mlp_grad,
+// then Adam, then launch_mlp_cast's copies); step = Adam's t after this step
+hipError_t launch_train_step(const MlpView& m, const float* x, const uint32_t* t, const float* w, uint32_t B,
+                             float* partial, float* loss, float* grad, float* P, float* mo, float* ve,
+                             const MlpBuffers& wb, float lr, uint32_t step, hipStream_t s);
 hipError_t launch_mlp_cast(const float* P, const MlpBuffers& b, hipStream_t s);
 hipError_t launch_train_gather(const FrameArgs& a, const uint32_t* clusterOf, uint32_t n, uint32_t salt, float* x,
                                uint32_t* t, float* w, hipStream_t s);

@@ -615,18 +615,44 @@ __global__ __launch_bounds__(256) void k_grad_reduce(const float* __restrict__ p
     if (wv == 0 && p < (uint32_t)kMlpParams) grad[p] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
-// Adam (Kingma & Ba 2015, Algorithm 1) on the FP32 master copy
-__global__ __launch_bounds__(256) void k_adam(float* __restrict__ P, const float* __restrict__ g, float* __restrict__ mo,
-                                              float* __restrict__ ve, float lr, float b1, float b2, float eps,
-                                              float c1, float c2)
+// One training step's update, fused: for 64 parameters per block, the
+// chunk gradients summed exactly as k_grad_reduce sums them, the Adam
+// update (Kingma & Ba 2015, Algorithm 1) on the FP32 master copy, and the
+// BF16 copies of the updated weights that k_mlp_cast writes (W1b's zero
+// padding never changes and is not written).
+__global__ __launch_bounds__(256) void k_grad_adam(const float* __restrict__ partial, uint32_t chunks,
+                                                   float* __restrict__ grad, float* __restrict__ P,
+                                                   float* __restrict__ mo, float* __restrict__ ve, MlpBuffers wb,
+                                                   float lr, float b1, float b2, float eps, float c1, float c2)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= (uint32_t)kMlpParams) return;
-    const float gi = g[i];
-    const float mi = b1 * mo[i] + (1.0f - b1) * gi;
-    const float vi = b2 * ve[i] + (1.0f - b2) * gi * gi;
-    mo[i] = mi, ve[i] = vi;
-    P[i] = P[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    __shared__ float part[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t p = blockIdx.x * 64u + lane;
+    float sum = 0.0f;
+    if (p < (uint32_t)kMlpParams)
+        for (uint32_t c = wv; c < chunks; c += 4u) sum += partial[(size_t)c * kMlpParams + p];
+    part[wv][lane] = sum;
+    __syncthreads();
+    if (wv != 0 || p >= (uint32_t)kMlpParams) return;
+    const float gi = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    grad[p] = gi;
+    const float mi = b1 * mo[p] + (1.0f - b1) * gi;
+    const float vi = b2 * ve[p] + (1.0f - b2) * gi * gi;
+    mo[p] = mi, ve[p] = vi;
+    const float np = P[p] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    P[p] = np;
+    const __bf16 nb = to_bf16(np);
+    if (p < (uint32_t)kOffB1) {
+        wb.W1b[(p / 9u) * 16u + p % 9u] = nb;
+    } else if (p >= (uint32_t)kOffW2 && p < (uint32_t)kOffB2) {
+        const uint32_t i = p - kOffW2, o = i / 64u, k = i % 64u;
+        wb.W2b[i] = nb;
+        wb.W2Tb[k * 64u + o] = nb;
+    } else if (p >= (uint32_t)kOffW3 && p < (uint32_t)kOffB3) {
+        const uint32_t i = p - kOffW3, o = i / 64u, k = i % 64u;
+        wb.W3b[i] = nb;
+        wb.W3Tb[k * 256u + o] = nb;
+    }
 }
 
 // BF16 copies of the master weights
@@ -725,11 +751,17 @@ hipError_t launch_mlp_grad(const MlpView& m, const float* x, const uint32_t* t, 
     return hipGetLastError();
 }
 
-hipError_t launch_adam(float* P, const float* g, float* mo, float* ve, float lr, uint32_t t, hipStream_t s)
+hipError_t launch_train_step(const MlpView& m, const float* x, const uint32_t* t, const float* w, uint32_t B,
+                             float* partial, float* loss, float* grad, float* P, float* mo, float* ve,
+                             const MlpBuffers& wb, float lr, uint32_t step, hipStream_t s)
 {
+    if (B == 0) return hipSuccess;
+    const uint32_t chunks = (B + 63) / 64;
     const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
-    hipLaunchKernelGGL(k_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, P, g, mo, ve, lr, b1, b2, 1e-8f, c1, c2);
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(chunks), dim3(256), 0, s, m, x, t, w, B, 1.0f / (float)B, partial, loss);
+    hipLaunchKernelGGL(k_grad_adam, dim3((kMlpParams + 63) / 64), dim3(256), 0, s, partial, chunks, grad, P, mo, ve,
+                       wb, lr, b1, b2, 1e-8f, c1, c2);
     return hipGetLastError();
 }
 

@@ -444,11 +444,11 @@ int train_alloc(vr_ctx* c, uint32_t n)
 int train_step(vr_ctx* c, uint32_t first, uint32_t n, float lr, double* loss)
 {
     const uint32_t chunks = (n + 63) / 64;
-    HIPCHK(c, launch_mlp_grad(mlp_view(c), c->trainX + (size_t)first * 9, c->trainT + first, c->trainW + first, n,
-                              c->partial, c->chunkLoss, c->mlpG, c->stream));
     c->adamStep++;
-    HIPCHK(c, launch_adam(c->mlpP, c->mlpG, c->mlpM, c->mlpV, lr, c->adamStep, c->stream));
-    HIPCHK(c, launch_mlp_cast(c->mlpP, MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, c->stream));
+    HIPCHK(c, launch_train_step(mlp_view(c), c->trainX + (size_t)first * 9, c->trainT + first, c->trainW + first, n,
+                                c->partial, c->chunkLoss, c->mlpG, c->mlpP, c->mlpM, c->mlpV,
+                                MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, lr, c->adamStep,
+                                c->stream));
     if (loss) {
         std::vector<float> l(chunks);
         HIPCHK(c, hipMemcpyAsync(l.data(), c->chunkLoss, (size_t)chunks * 4, hipMemcpyDeviceToHost, c->stream));
