@@ -250,8 +250,9 @@ int vr_stage_times(vr_ctx* ctx, float ms[VR_NUM_STAGES]);
 
 /* Rays traced by the following frames (a diagnostic with no counterpart in
  * the reference, which traces 3 per pixel: primary, candidate and shading
- * shadow rays).  vr_set_ray_counting(ctx, 1) zeroes the counters and counts
- * from the next frame on, with one atomic add per traced ray; 0 stops.
+ * shadow rays).  vr_set_ray_counting(ctx, 1) waits for the queued frames,
+ * zeroes the counters and counts from the next frame on, with one atomic
+ * add per traced ray; 0 stops.
  * vr_read_ray_counts waits for the queued frames and returns the sums:
  * [0] primary rays (with renderAlbedo's translucency continuations),
  * [1] the candidate pass's shadow rays, [2] shading's (those the candidate
@@ -293,9 +294,11 @@ int vr_trace_rays_noalloc(vr_ctx* ctx, const float* origins, const float* dirs, 
  * from the defensive mixture m = (1 - eps) q + eps P (P_c = the cluster's
  * share of the total light power), then a light drawn inside the cluster in
  * proportion to its power; its source pdf m_c * pmf_i / P_c enters the RIS
- * weight p-hat/p (eps = 1 is exactly the power source).  The 32 candidates
- * of a pixel are stratified over m.  BF16 MFMA with FP32 accumulation for
- * inference and training; FP32 master weights and Adam. */
+ * weight p-hat/p (eps = 1 is exactly the power source).  The n candidates
+ * of a pixel (n = min(lights, numberOfCandidates)) take the targets
+ * (j + u_j)/n of m's CDF, u_j uniform: stratified, one per stratum.  BF16
+ * MFMA with FP32 accumulation for inference and training;
+ * FP32 master weights and Adam. */
 #define VR_MLP_PARAMS 21440u  /* W1[64][9] b1[64] W2[64][64] b2[64] W3[256][64] b3[256] */
 
 /* cluster of every uploaded light (0..nclusters-1, nclusters <= 256); must

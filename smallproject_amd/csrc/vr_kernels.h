@@ -52,10 +52,9 @@ struct FrameArgs {
     uint32_t misSpatial; // spatial merge with MIS weights (VR_FLAG_MIS_SPATIAL)
     uint32_t waves;      // occupancy target of this stage's kernel (0: kDefaultWaves)
     // learned source: candidate slot j of pixel i at [j * W*H + i] holds a
-    // cluster and its mixture mass; inside the cluster the light is drawn
-    // with the cluster's power alias table
-    const uint32_t* candIdx;
-    const float* candPdf;
+    // cluster and the bits of its mixture mass; inside the cluster the light
+    // is drawn with the cluster's power alias table
+    const uint2* candSlot;
     const uint32_t* clusterStart;  // [257] offsets into the cluster-ordered light list
     // bucket j of the list (cluster c's buckets are clusterStart[c]..): the
     // alias table's (prob bits, light j, light of alias j, alias j local

@@ -372,8 +372,9 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
                 // learned proposal: the slot's cluster and mixture mass; the
                 // light inside the cluster by power (one draw for bucket and
                 // alias, as pick_light), pdf = m_c * pmf_i / P_c
-                const uint32_t c = a.candIdx[i * plane + idx];
-                const float mc = a.candPdf[i * plane + idx];
+                const uint2 slot = a.candSlot[i * plane + idx];
+                const uint32_t c = slot.x;
+                const float mc = __uint_as_float(slot.y);
                 const uint32_t c0 = c < 256u ? a.clusterStart[c] : 0u;
                 const uint32_t cnt = c < 256u ? a.clusterStart[c + 1] - c0 : 0u;
                 const float u = rnd(seed);

@@ -58,14 +58,12 @@ __host__ __device__ inline uint32_t proposal_salt(uint32_t R0) { return R0 * 0x9
 struct ProposalArgs {
     FrameArgs a;
     MlpView m;
-    uint32_t* candIdx;
-    float* candPdf;
+    uint2* candSlot;
     uint32_t salt;
-    void* ptrs[5];
+    void* ptrs[4];
 };
 hipError_t proposal_node(ProposalArgs* pa, hipKernelNodeParams* p);
-hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint32_t* candIdx, float* candPdf, uint32_t salt,
-                           hipStream_t s);
+hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint2* candSlot, uint32_t salt, hipStream_t s);
 hipError_t launch_mlp_probs(const MlpView& m, const float* x, uint32_t B, float* out, hipStream_t s);
 hipError_t launch_mlp_grad(const MlpView& m, const float* x, const uint32_t* t, const float* w, uint32_t B,
                            float* partial, float* loss, float* grad, hipStream_t s);

@@ -255,9 +255,10 @@ __device__ __forceinline__ void softmax_infer(const MlpView& m, const float* sBi
 // slots 0..n-1), so the RIS weight p-hat/p stays unbiased.  The lane
 // pair of a pixel splits the CDF (the lower lane holds the clusters with
 // (c & 4) == 0) and each lane walks its part once in cluster order, writing
-// slot j as (cluster, m_c); the candidate pass draws the light inside the
-// cluster in proportion to its power, so the source pdf of light i is
-// m_c * pmf_i / P_c (eps = 1 gives the plain power source exactly).
+// slot j as (cluster, m_c bits) in one 8-byte store; the candidate pass
+// draws the light inside the cluster in proportion to its power, so the
+// source pdf of light i is m_c * pmf_i / P_c (eps = 1 gives the plain power
+// source exactly).
 // ---------------------------------------------------------------------------
 struct ProposalLds {
     InferLds w[4];
@@ -265,9 +266,8 @@ struct ProposalLds {
     float power[kMaxClusters];
 };
 
-__device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView& m, uint32_t* __restrict__ candIdx,
-                                              float* __restrict__ candPdf, uint32_t salt, int tx, int ty,
-                                              ProposalLds& L)
+__device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView& m, uint2* __restrict__ candSlot,
+                                              uint32_t salt, int tx, int ty, ProposalLds& L)
 {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     InferLds& s = L.w[wv];
@@ -338,6 +338,7 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
         const float base = upper ? massLo : 0.0f;
         int k = kBeg;
         float tk = k < kEnd ? target(k) : 2.0f;  // next target of this lane
+        uint2* slot = candSlot + (size_t)kBeg * plane + idx;  // slot k of the pixel
         float cdf = base;
         uint32_t lastC = 0xffffffffu;
         float lastM = 0.0f;
@@ -351,10 +352,8 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
                 cdf = cdf + mc;
                 if (mc > 0.0f) {
                     while (tk < cdf) {
-                        if (valid) {
-                            candIdx[(size_t)k * plane + idx] = c;
-                            candPdf[(size_t)k * plane + idx] = mc;
-                        }
+                        if (valid) *slot = make_uint2(c, __float_as_uint(mc));
+                        slot += plane;
                         k++;
                         tk = k < kEnd ? target(k) : 2.0f;
                     }
@@ -365,21 +364,17 @@ __device__ __forceinline__ void proposal_tile(const FrameArgs& a, const MlpView&
         // rounding can leave targets just above the walked mass: they take
         // the last cluster with mass; a lane without any mass marks them
         // empty (candidate skipped with weight 0)
-        for (; k < kEnd; k++)
-            if (valid) {
-                candIdx[(size_t)k * plane + idx] = lastC;
-                candPdf[(size_t)k * plane + idx] = lastM;
-            }
+        for (; k < kEnd; k++, slot += plane)
+            if (valid) *slot = make_uint2(lastC, __float_as_uint(lastM));
     }
 }
 
-__global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint32_t* __restrict__ candIdx,
-                                                  float* __restrict__ candPdf, uint32_t salt)
+__global__ __launch_bounds__(256) void k_proposal(FrameArgs a, MlpView m, uint2* __restrict__ candSlot, uint32_t salt)
 {
     __shared__ ProposalLds L;
     int tx, ty;
     tile_of_block(a.tileOrder, tx, ty);
-    proposal_tile(a, m, candIdx, candPdf, salt, tx, ty, L);
+    proposal_tile(a, m, candSlot, salt, tx, ty, L);
 }
 
 
@@ -713,17 +708,15 @@ __global__ __launch_bounds__(256) void k_train_gather(FrameArgs a, const uint32_
 }
 
 // ---------------------------------------------------------------------------
-hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint32_t* candIdx, float* candPdf, uint32_t salt,
-                           hipStream_t s)
+hipError_t launch_proposal(const FrameArgs& a, const MlpView& m, uint2* candSlot, uint32_t salt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_proposal, dim3((a.W + 15) / 16, (a.H + 15) / 16), dim3(256), 0, s, a, m, candIdx, candPdf, salt);
+    hipLaunchKernelGGL(k_proposal, dim3((a.W + 15) / 16, (a.H + 15) / 16), dim3(256), 0, s, a, m, candSlot, salt);
     return hipGetLastError();
 }
 
 hipError_t proposal_node(ProposalArgs* pa, hipKernelNodeParams* p)
 {
-    pa->ptrs[0] = &pa->a, pa->ptrs[1] = &pa->m, pa->ptrs[2] = &pa->candIdx, pa->ptrs[3] = &pa->candPdf;
-    pa->ptrs[4] = &pa->salt;
+    pa->ptrs[0] = &pa->a, pa->ptrs[1] = &pa->m, pa->ptrs[2] = &pa->candSlot, pa->ptrs[3] = &pa->salt;
     *p = hipKernelNodeParams{};
     p->func = reinterpret_cast<void*>(&k_proposal);
     p->gridDim = dim3((pa->a.W + 15) / 16, (pa->a.H + 15) / 16);

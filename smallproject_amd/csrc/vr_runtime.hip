@@ -138,8 +138,7 @@ struct vr_ctx {
     uint32_t clusterDraw = VR_CLUSTER_DRAW_POWER;  // the light inside a drawn cluster
     std::vector<vr_light> hostLights;   // the uploaded light list
     uint32_t nclusters = 0, clusterLightCap = 0, clusteredLights = 0;
-    uint32_t* candIdx = nullptr;  // [kCandidates][W*H]
-    float* candPdf = nullptr;
+    uint2* candSlot = nullptr;  // [kCandidates][W*H]: cluster, mass bits
     float* trainX = nullptr;      // training batch
     uint32_t* trainT = nullptr;
     float* trainW = nullptr;
@@ -363,8 +362,7 @@ FrameArgs frame_args(const vr_ctx* c, const vr_render_params* p)
     a.shadowSteps = (c->cfg.quirks & VR_QUIRK_SHADOW_STEP_CAP) ? 128 : 999999;
     a.tileOrder = c->cfg.flags & 3u;
     a.misSpatial = (c->cfg.flags & VR_FLAG_MIS_SPATIAL) ? 1u : 0u;
-    a.candIdx = c->candIdx;
-    a.candPdf = c->candPdf;
+    a.candSlot = c->candSlot;
     a.clusterStart = c->clusterStart;
     a.clusterPick = c->clusterPick;
     a.clusterPickShare = c->clusterPickShare;
@@ -389,12 +387,12 @@ int mlp_alloc(vr_ctx* c)
                       ok(dalloc(&c->mlpW2Tb, 64 * 64)) && ok(dalloc(&c->mlpW3b, 256 * 64)) &&
                       ok(dalloc(&c->mlpW3Tb, 256 * 64)) && ok(dalloc(&c->clusterStart, kMaxClusters + 1)) &&
                       ok(dalloc(&c->clusterPower, kMaxClusters)) &&
-                      ok(dalloc(&c->candIdx, npx * kCandidates)) && ok(dalloc(&c->candPdf, npx * kCandidates));
+                      ok(dalloc(&c->candSlot, npx * kCandidates));
     if (!good) {
         // free what was allocated, so that a later call starts clean
         for (void** q : {(void**)&c->mlpP, (void**)&c->mlpM, (void**)&c->mlpV, (void**)&c->mlpG, (void**)&c->mlpW1b,
                          (void**)&c->mlpW2b, (void**)&c->mlpW2Tb, (void**)&c->mlpW3b, (void**)&c->mlpW3Tb,
-                         (void**)&c->clusterStart, (void**)&c->clusterPower, (void**)&c->candIdx, (void**)&c->candPdf})
+                         (void**)&c->clusterStart, (void**)&c->clusterPower, (void**)&c->candSlot})
             dfree(*q), *q = nullptr;
         return fail(c, VR_ERR_NOMEM, "learned proposal: device allocation failed");
     }
@@ -548,7 +546,7 @@ int vr_destroy(vr_ctx* c)
     dfree(c->mlpW1b), dfree(c->mlpW2b), dfree(c->mlpW2Tb), dfree(c->mlpW3b), dfree(c->mlpW3Tb);
     dfree(c->clusterStart), dfree(c->clusterOf), dfree(c->clusterPick), dfree(c->clusterPickShare);
     dfree(c->clusterPower);
-    dfree(c->candIdx), dfree(c->candPdf);
+    dfree(c->candSlot);
     dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
     dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget), dfree(c->claim);
     for (int i = 0; i < 2 * VR_NUM_STAGES; i++)
@@ -706,8 +704,7 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         // the proposal runs after the primary hits and is timed with the
         // candidate stage
         if (s == VR_STAGE_INITIAL && learned)
-            HIPCHK(c, launch_proposal(as, mlp_view(c), c->candIdx, c->candPdf, proposal_salt(p->R0),
-                                      c->stream));
+            HIPCHK(c, launch_proposal(as, mlp_view(c), c->candSlot, proposal_salt(p->R0), c->stream));
         HIPCHK(c, launch_stage(s, as, st));
         HIPCHK(c, hipEventRecord(c->ev[slot][2 * s + 1], st));
         if (s == VR_STAGE_INITIAL) {
@@ -797,7 +794,15 @@ int vr_read_frame(vr_ctx* c, float* rgba)
 int vr_set_ray_counting(vr_ctx* c, int enable)
 {
     if (!c) return VR_ERR_ARG;
-    if (enable) HIPCHK(c, hipMemsetAsync(c->rays, 0, kRayKinds * 8, c->stream));
+    if (enable) {
+        // the queued frames end first: a counted frame's primary pass may
+        // run on the side stream, which a zeroing queued on the context's
+        // stream would not be ordered against
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->side) HIPCHK(c, hipStreamSynchronize(c->side));
+        HIPCHK(c, hipMemsetAsync(c->rays, 0, kRayKinds * 8, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     c->countRays = enable != 0;
     return VR_OK;
 }
@@ -1171,8 +1176,12 @@ int vr_read_candidates(vr_ctx* c, uint32_t* idx, float* pdf)
     if (!c->mlpReady) return fail(c, VR_ERR_STATE, "vr_read_candidates: no learned proposal");
     const size_t n = (size_t)c->cfg.width * c->cfg.height * kCandidates;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (idx) HIPCHK(c, hipMemcpy(idx, c->candIdx, n * 4, hipMemcpyDeviceToHost));
-    if (pdf) HIPCHK(c, hipMemcpy(pdf, c->candPdf, n * 4, hipMemcpyDeviceToHost));
+    std::vector<uint2> slots(n);
+    HIPCHK(c, hipMemcpy(slots.data(), c->candSlot, n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        if (idx) idx[i] = slots[i].x;
+        if (pdf) std::memcpy(&pdf[i], &slots[i].y, 4);
+    }
     return VR_OK;
 }
 
@@ -1549,7 +1558,7 @@ int vr_batch_render(vr_batch* b, const vr_render_params* params)
         sl.chunks[i / b->chunkViews].v[i % b->chunkViews] = a;
         if (c->cfg.source_pdf == VR_SOURCE_LEARNED) {
             ProposalArgs& pa = sl.props[li++];
-            pa.a = a, pa.m = mlp_view(c), pa.candIdx = c->candIdx, pa.candPdf = c->candPdf;
+            pa.a = a, pa.m = mlp_view(c), pa.candSlot = c->candSlot;
             pa.salt = proposal_salt(params[i].R0);
         }
         // uploads queued on the view's own stream (commits, lights) first
