@@ -191,13 +191,17 @@ class Renderer:
                   "vr_mlp_step")
         return loss.value
 
-    def mlp_train(self, nsamples: int = 1 << 16, lr: float = 1e-3, steps: int = 1, source=None) -> float:
+    def mlp_train(self, nsamples: int = 1 << 16, lr: float = 1e-3, steps: int = 1, source=None,
+                  want_loss: bool = True):
         """Online training step from the last frame of `source` (a Renderer
-        on the same light list, e.g. the plain-source one) or of this one."""
+        on the same light list, e.g. the plain-source one) or of this one.
+        Returns the last minibatch's loss, which waits for the steps; with
+        want_loss=False the steps stay queued and None is returned."""
         loss = C.c_double()
         src = None if source is None else source.h
-        self._chk(lib().vr_mlp_train(self.h, src, nsamples, C.c_float(lr), steps, C.byref(loss)), "vr_mlp_train")
-        return loss.value
+        self._chk(lib().vr_mlp_train(self.h, src, nsamples, C.c_float(lr), steps,
+                                     C.byref(loss) if want_loss else None), "vr_mlp_train")
+        return loss.value if want_loss else None
 
     def set_cluster_draw(self, mode: int):
         """Light inside a drawn cluster: by power (CLUSTER_DRAW_POWER) or

@@ -78,6 +78,11 @@ struct vr_ctx {
     hipStream_t side = nullptr;
     bool counted = false;  // included in g_liveContexts
     hipEvent_t initDone = nullptr, primDone = nullptr;
+    // The learned proposal follows the primary pass on `side` too; it reads
+    // the network's weights as of their last update, recorded here on
+    // `stream` after every training step (other weight and cluster-table
+    // changes return only once done).
+    hipEvent_t weightsReady = nullptr;
     bool initRecorded = false;
     uint64_t worldEpoch = 0, epochAtInit = 0;
     // start and end events of each stage of the last kRing frames
@@ -447,6 +452,7 @@ int train_step(vr_ctx* c, uint32_t first, uint32_t n, float lr, double* loss)
                                 c->partial, c->chunkLoss, c->mlpG, c->mlpP, c->mlpM, c->mlpV,
                                 MlpBuffers{c->mlpW1b, c->mlpW2b, c->mlpW2Tb, c->mlpW3b, c->mlpW3Tb}, lr, c->adamStep,
                                 c->stream));
+    HIPCHK(c, hipEventRecord(c->weightsReady, c->stream));
     if (loss) {
         std::vector<float> l(chunks);
         HIPCHK(c, hipMemcpyAsync(l.data(), c->chunkLoss, (size_t)chunks * 4, hipMemcpyDeviceToHost, c->stream));
@@ -493,7 +499,11 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     auto ok = [&](hipError_t e) { return e == hipSuccess; };
     bool good = ok(hipGetDevice(&c->device)) && ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) &&
                 ok(hipEventCreateWithFlags(&c->initDone, hipEventDisableTiming)) &&
-                ok(hipEventCreateWithFlags(&c->primDone, hipEventDisableTiming));
+                ok(hipEventCreateWithFlags(&c->primDone, hipEventDisableTiming)) &&
+                ok(hipEventCreateWithFlags(&c->weightsReady, hipEventDisableTiming));
+    // recorded once up front, so that waiting on it before any training
+    // step is defined
+    good = good && ok(hipEventRecord(c->weightsReady, c->stream));
     for (int f = 0; good && f < kRing; f++)
         for (int i = 0; good && i < 2 * VR_NUM_STAGES; i++) good = ok(hipEventCreate(&c->ev[f][i]));
     good = good && ok(dalloc(&c->grid, VR_GRIDSIZE)) && ok(dalloc(&c->uber, 1024)) &&
@@ -554,6 +564,7 @@ int vr_destroy(vr_ctx* c)
             if (c->ev[f][i]) (void)hipEventDestroy(c->ev[f][i]);
     if (c->initDone) (void)hipEventDestroy(c->initDone);
     if (c->primDone) (void)hipEventDestroy(c->primDone);
+    if (c->weightsReady) (void)hipEventDestroy(c->weightsReady);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -696,14 +707,22 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         as.tileOrder = (c->cfg.flags >> (2 * s)) & 3u;
         as.waves = c->stageWaves[s];
         hipStream_t st = s == VR_STAGE_PRIMARY ? ps : c->stream;
+        // the proposal runs after the primary hits and is timed with the
+        // candidate stage: beside the previous frame's last stage too when
+        // the primary pass is (it writes the slots that only the previous
+        // frame's candidate pass read, and waits for the last training step)
+        const bool propose = s == VR_STAGE_INITIAL && learned;
         if (s == VR_STAGE_INITIAL && overlap) {
+            if (propose) {
+                HIPCHK(c, hipStreamWaitEvent(c->side, c->weightsReady, 0));
+                HIPCHK(c, hipEventRecord(c->ev[slot][2 * s], c->side));
+                HIPCHK(c, launch_proposal(as, mlp_view(c), c->candSlot, proposal_salt(p->R0), c->side));
+            }
             HIPCHK(c, hipEventRecord(c->primDone, c->side));
             HIPCHK(c, hipStreamWaitEvent(c->stream, c->primDone, 0));
         }
-        HIPCHK(c, hipEventRecord(c->ev[slot][2 * s], st));
-        // the proposal runs after the primary hits and is timed with the
-        // candidate stage
-        if (s == VR_STAGE_INITIAL && learned)
+        if (!(propose && overlap)) HIPCHK(c, hipEventRecord(c->ev[slot][2 * s], st));
+        if (propose && !overlap)
             HIPCHK(c, launch_proposal(as, mlp_view(c), c->candSlot, proposal_salt(p->R0), c->stream));
         HIPCHK(c, launch_stage(s, as, st));
         HIPCHK(c, hipEventRecord(c->ev[slot][2 * s + 1], st));

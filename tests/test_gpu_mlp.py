@@ -317,6 +317,40 @@ def test_dynamic_run_falls_back_by_its_rule(gpu):
         assert c["mse_plain"] > 0 and c["mse_learned_context"] > 0
 
 
+def test_learned_back_to_back_frames_equal_synchronised_frames(gpu):
+    """A lone context queues a learned frame's proposal behind its primary
+    pass on its second stream, beside the previous frame's last stage, once
+    the last queued training step is done.  Frames queued back to back, with
+    training steps left queued between some of them, must come out bit for
+    bit as the same frames each waited for (slots, hits, final reservoirs,
+    radiance)."""
+    outs = []
+    for wait in (False, True):
+        sc = dataclasses.replace(configs.CONFIGS["c1_point_128"], source=_lib.SOURCE_LEARNED)
+        w = configs.build_world(sc, 160, 96)
+        w.force_sync()
+        r = w.renderer
+        r.set_light_clusters(w.cluster_lights(16, 8, 1), 16)
+        r.mlp_set_params(random_params(np.random.default_rng(31), 2.0))
+        for f in range(6):
+            pos, tgt = sc.camera(f)
+            w.look_at(pos, tgt)
+            w.render()
+            if f in (1, 3):
+                r.mlp_train(1 << 12, 1e-3, 2, want_loss=False)
+            if wait:
+                r.sync()
+        idx, mass = r.candidates()
+        hw, d, sd = r.hits()
+        o = [idx, mass.view(np.uint32), hw, d.view(np.uint32), sd, r.frame().view(np.uint32)]
+        a, b, m = r.reservoirs(0)
+        o += [a.view(np.uint32), b.view(np.uint32), m]
+        outs.append(o)
+        w.close()
+    for k, (g, e) in enumerate(zip(*outs)):
+        assert np.array_equal(g, e), f"output {k}: {np.mean(g != e):.6f} differ"
+
+
 def test_trained_proposal_beats_the_plain_source(gpu):
     """What the trained network gains over the method without it: on the
     many-light configuration (65,536 emitters in 256 clusters, at 480x270),
