@@ -300,7 +300,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-quality", action="store_true", help="skip the equal-time MSE evaluation")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--tile-order", default="auto", choices=["auto", "linear", "xcd_rows", "xcd_cols"],
+    ap.add_argument("--tile-order", default="auto", choices=["auto", "linear", "xcd_rows", "xcd_cols", "xcd_blocks"],
                     help="workgroup->tile order; auto times each on the first frames after warm-up")
     ap.add_argument("--spatial", default="biased", choices=["biased", "mis"],
                     help="spatial merge: the reference's (biased) or MIS-weighted with a visibility ray per neighbour")

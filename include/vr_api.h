@@ -74,6 +74,7 @@ extern "C" {
 #define VR_TILES_LINEAR   0u  /* row-major */
 #define VR_TILES_XCD_ROWS 1u  /* each of the 8 XCDs takes one band of tile rows */
 #define VR_TILES_XCD_COLS 2u  /* each XCD takes one band of tile columns, row by row */
+#define VR_TILES_XCD_BLOCKS 3u /* each XCD takes whole 4x4-tile blocks, dealt round-robin */
 
 /* struct Light, template/common.h:134-139 */
 typedef struct vr_light {

@@ -22,7 +22,7 @@ CANDIDATES = 32
 QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
 QUIRKS_REFERENCE = 7
 TRACE_ALL, TRACE_OPAQUE = 0, 1
-TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS = 0, 1, 2
+TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS, TILES_XCD_BLOCKS = 0, 1, 2, 3
 FLAG_MIS_SPATIAL, MIS_MAX_TAPS = 1 << 8, 32
 MAX_VIEWS = 64
 NOALLOC_MAX_RAYS = 1 << 18  # VR_NOALLOC_MAX_RAYS

@@ -16,7 +16,8 @@ from __future__ import annotations
 
 from . import _lib
 
-ORDERS = {"linear": _lib.TILES_LINEAR, "xcd_rows": _lib.TILES_XCD_ROWS, "xcd_cols": _lib.TILES_XCD_COLS}
+ORDERS = {"linear": _lib.TILES_LINEAR, "xcd_rows": _lib.TILES_XCD_ROWS, "xcd_cols": _lib.TILES_XCD_COLS,
+          "xcd_blocks": _lib.TILES_XCD_BLOCKS}
 
 
 def tune_tile_order(renderer, render_frame, frames: int = 8, rounds: int = 2, margin: float = 0.02):

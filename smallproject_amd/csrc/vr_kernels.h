@@ -75,10 +75,17 @@ struct FrameArgs {
 //                     b % 8 row by row, so the tiles in flight still form
 //                     one strip and each XCD's part of it is contiguous;
 //                     other widths fall back to the linear order
+//   VR_TILES_XCD_BLOCKS the tiles taken in blocks of 4x4 (64x64 pixels, row
+//                     by row within a block, blocks row by row, the edge
+//                     blocks clipped); each XCD takes 16 consecutive tiles
+//                     of that sequence at a time, XCD after XCD, so a
+//                     pixel's spatial taps mostly fall in tiles of its own
+//                     XCD while all XCDs sweep the image together (the tail
+//                     beyond a multiple of 128 keeps its own position)
 // Every order is a bijection on [0, n).
 __device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
 {
-    const uint32_t nbx = gridDim.x, nb = gridDim.x * gridDim.y;
+    const uint32_t nbx = gridDim.x, nby = gridDim.y, nb = gridDim.x * gridDim.y;
     const uint32_t b = blockIdx.y * nbx + blockIdx.x;
     uint32_t t = b;
     if (order == 1u) {
@@ -87,6 +94,20 @@ __device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
     } else if (order == 2u && (nbx & 7u) == 0u) {
         const uint32_t cw = nbx >> 3, i = b >> 3;
         t = (i / cw) * nbx + (b & 7u) * cw + i % cw;
+    } else if (order == 3u) {
+        // position p in the block sequence: XCD b % 8 takes run b / 128 * 8
+        // + b % 8 of 16 positions, one position per round of 8 workgroups
+        const uint32_t full = nb & ~127u;
+        const uint32_t p = b < full ? (b & ~127u) + ((b & 7u) << 4) + ((b >> 3) & 15u) : b;
+        // p -> tile: block rows of 4 tile rows (the last may be shorter),
+        // blocks of 4 columns (the last may be narrower), row-major inside
+        const uint32_t R = p / (nbx * 4u), q = p - R * nbx * 4u;
+        const uint32_t h = min(4u, nby - 4u * R);
+        const uint32_t C = q / (4u * h), in = q - C * 4u * h;
+        const uint32_t w = min(4u, nbx - 4u * C);
+        tx = (int)(4u * C + in % w);
+        ty = (int)(4u * R + in / w);
+        return;
     }
     tx = (int)(t % nbx);
     ty = (int)(t / nbx);

@@ -491,7 +491,7 @@ int vr_create(const vr_config* cfg, vr_ctx** out)
     if (cfg->emit_lut[0] != 0.0f) return VR_ERR_ARG;
     if (cfg->flags & ~(((1u << (2 * VR_NUM_STAGES)) - 1u) | VR_FLAG_MIS_SPATIAL)) return VR_ERR_ARG;
     for (int s = 0; s < VR_NUM_STAGES; s++)
-        if (((cfg->flags >> (2 * s)) & 3u) > VR_TILES_XCD_COLS) return VR_ERR_ARG;
+        if (((cfg->flags >> (2 * s)) & 3u) > VR_TILES_XCD_BLOCKS) return VR_ERR_ARG;
     vr_ctx* c = new (std::nothrow) vr_ctx();
     if (!c) return VR_ERR_NOMEM;
     c->cfg = *cfg;
@@ -758,7 +758,7 @@ int vr_set_stage_waves(vr_ctx* c, int32_t stage, uint32_t waves)
 int vr_set_tile_order(vr_ctx* c, int32_t stage, uint32_t order)
 {
     if (!c) return VR_ERR_ARG;
-    if (order > VR_TILES_XCD_COLS) return fail(c, VR_ERR_ARG, "vr_set_tile_order: unknown order");
+    if (order > VR_TILES_XCD_BLOCKS) return fail(c, VR_ERR_ARG, "vr_set_tile_order: unknown order");
     if (stage < -1 || stage >= VR_NUM_STAGES) return fail(c, VR_ERR_ARG, "vr_set_tile_order: unknown stage");
     for (int s = 0; s < VR_NUM_STAGES; s++)
         if (stage < 0 || stage == s) c->cfg.flags = (c->cfg.flags & ~(3u << (2 * s))) | (order << (2 * s));

@@ -180,11 +180,11 @@ def test_mis_spatial_merge_rejects_too_many_taps(gpu):
 @pytest.mark.parametrize("size", [(256, 144), (200, 120)])
 def test_tile_orders_render_identical_frames(gpu, size):
     """The workgroup->tile order is a bijection: every order renders the same
-    bits (200 px = 13 tiles per row exercises the row-band tail and the
-    column-band fallback)."""
+    bits (200 px = 13 tiles per row exercises the row-band tail, the
+    column-band fallback and clipped 4x4 blocks)."""
     sc = configs.CONFIGS["c1_point_128"]
     outs = []
-    for order in (_lib.TILES_LINEAR, _lib.TILES_XCD_ROWS, _lib.TILES_XCD_COLS):
+    for order in (_lib.TILES_LINEAR, _lib.TILES_XCD_ROWS, _lib.TILES_XCD_COLS, _lib.TILES_XCD_BLOCKS):
         w = configs.build_world(sc, *size)
         w.force_sync()
         r = w.renderer
