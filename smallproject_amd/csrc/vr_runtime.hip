@@ -1195,11 +1195,17 @@ int vr_read_candidates(vr_ctx* c, uint32_t* idx, float* pdf)
     if (!c->mlpReady) return fail(c, VR_ERR_STATE, "vr_read_candidates: no learned proposal");
     const size_t n = (size_t)c->cfg.width * c->cfg.height * kCandidates;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<uint2> slots(n);
-    HIPCHK(c, hipMemcpy(slots.data(), c->candSlot, n * 8, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n; i++) {
-        if (idx) idx[i] = slots[i].x;
-        if (pdf) std::memcpy(&pdf[i], &slots[i].y, 4);
+    // the 8-byte slots split into the two output arrays, through a bounded
+    // host buffer (a 1080p frame holds 66 M slots)
+    const size_t chunk = (size_t)1 << 20;
+    std::vector<uint2> slots(n < chunk ? n : chunk);
+    for (size_t i0 = 0; i0 < n; i0 += chunk) {
+        const size_t m = n - i0 < chunk ? n - i0 : chunk;
+        HIPCHK(c, hipMemcpy(slots.data(), c->candSlot + i0, m * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < m; i++) {
+            if (idx) idx[i0 + i] = slots[i].x;
+            if (pdf) std::memcpy(&pdf[i0 + i], &slots[i].y, 4);
+        }
     }
     return VR_OK;
 }

@@ -76,7 +76,7 @@ def _reject_everything(w):
     _rejects(_lib.VR_ERR_ARG, r.set_stage_waves, 4)
     _rejects(_lib.VR_ERR_ARG, r.set_stage_waves, 9)
     _rejects(_lib.VR_ERR_ARG, r.set_stage_waves, 6, _lib.NUM_STAGES)
-    _rejects(_lib.VR_ERR_ARG, r.set_tile_order, _lib.TILES_XCD_COLS + 1, match="unknown order")
+    _rejects(_lib.VR_ERR_ARG, r.set_tile_order, _lib.TILES_XCD_BLOCKS + 1, match="unknown order")
     _rejects(_lib.VR_ERR_ARG, r.set_tile_order, _lib.TILES_LINEAR, _lib.NUM_STAGES, match="unknown stage")
 
     lights = r.read_lights(nl)
