@@ -11,8 +11,15 @@ in alternating order and each order's best round counts (robust to a one-off
 clock or queue hiccup).  An order must beat the linear order by more than
 `margin` to be chosen.  The occupancy targets are then searched the same
 way with the chosen orders, against each stage's default target.
+
+Stage times overlap (a frame's primary pass runs beside the previous
+frame's last stage), so per-stage choices can add up to a slower frame:
+`confirm` times whole frames of the tuned settings against the defaults
+and keeps the faster.
 """
 from __future__ import annotations
+
+import time
 
 from . import _lib
 
@@ -75,3 +82,32 @@ def tune_waves(renderer, render_frame, frames: int = 8, rounds: int = 2, margin:
         chosen[_lib.STAGE_NAMES[st]] = best
     return chosen, times
 
+
+
+def _apply(renderer, orders: dict, waves: dict):
+    for st, name in enumerate(_lib.STAGE_NAMES):
+        renderer.set_tile_order(ORDERS[orders.get(name, "linear")], st)
+        renderer.set_stage_waves(waves.get(name, 0), st)
+
+
+def confirm(renderer, render_frame, orders: dict, waves: dict, frames: int = 30, rounds: int = 3):
+    """Whole-frame wall time of the tuned settings (orders, waves: {stage
+    name: choice}) against the defaults (linear order, default targets) on
+    the same frames; keeps the faster settings set.  Returns (tuned kept,
+    {"tuned": ms, "default": ms}) with each side's best round."""
+    best = {"tuned": float("inf"), "default": float("inf")}
+    sides = [("tuned", orders, waves), ("default", {}, {})]
+    for rd in range(rounds):
+        for name, o, w in (sides if rd % 2 == 0 else sides[::-1]):
+            _apply(renderer, o, w)
+            render_frame(0)
+            renderer.sync()
+            t0 = time.perf_counter()
+            for i in range(frames):
+                render_frame(1 + i)
+            renderer.sync()
+            best[name] = min(best[name], (time.perf_counter() - t0) * 1e3 / frames)
+    keep = best["tuned"] < best["default"]
+    if keep:
+        _apply(renderer, orders, waves)
+    return keep, best
