@@ -310,6 +310,164 @@ __device__ uint32_t trace(const WorldView& w, f3 A, f3 B, int steps, uint32_t th
     return 0u;
 }
 
+// TraceOpaqueRay (trace.cl:205-282) for rays that go their own ways, as
+// shadow rays do.  trace() nests the brick walk inside the top-grid loop
+// inside the uber loop, so a wave runs each brick walk while the lanes that
+// are not inside a brick wait (on c2's shadow rays about 4 of 64 lanes take
+// each brick step).  Here the same traversal is cut into units - one visit
+// and step of an uber cell, of a top cell, or of a voxel - and each turn of
+// the wave's loop runs the unit of the level that most of the wave's lanes
+// are at; the others keep their state.  Every lane performs the operations
+// of trace() in the same order (each level keeps its own tm registers, which
+// trace() saves and restores around a descent), so the results are the same
+// bits.  `resume`: the lane's next unit at its level is the step that
+// follows a child level's exit (trace() steps the parent right after the
+// child loop ends).  A brick walk needs at most 22 steps (7 per axis and
+// the one that leaves), so trace()'s 24-step cap never ends one.
+__device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float& dist, uint32_t& side)
+{
+    if (fabsf(B.x) < 1e-8f) B.x = B.x < 0 ? -1e-8f : 1e-8f;
+    if (fabsf(B.y) < 1e-8f) B.y = B.y < 0 ? -1e-8f : 1e-8f;
+    if (fabsf(B.z) < 1e-8f) B.z = B.z < 0 ? -1e-8f : 1e-8f;
+    const f3 V = B;
+    const f3 rV = f3{1.0f / V.x, 1.0f / V.y, 1.0f / V.z};
+    const uint32_t ox = V.x > 0 ? 1u : 0u, oy = V.y > 0 ? 1u : 0u, oz = V.z > 0 ? 1u : 0u;
+    const uint32_t dx = V.x > 0 ? (1u << 20) : (0xfffu << 20);
+    const uint32_t dy = V.y > 0 ? (1u << 10) : 0xfffffc00u;
+    const uint32_t dz = V.z > 0 ? 1u : 0xffffffffu;
+    const f3 td = f3{fabsf(rV.x), fabsf(rV.y), fabsf(rV.z)};
+    float to = 0.0f;
+    uint32_t last = 0;
+    bool active = true;
+    if (A.x < 0 || A.y < 0 || A.z < 0 || A.x > 1024.0f || A.y > 1024.0f || A.z > 1024.0f) {
+        const float tx1 = -A.x * rV.x, tx2 = (1024.0f - A.x) * rV.x;
+        float tmin = fminf(tx1, tx2), tmax = fmaxf(tx1, tx2);
+        const float ty1 = -A.y * rV.y, ty2 = (1024.0f - A.y) * rV.y;
+        tmin = fmaxf(tmin, fminf(ty1, ty2)), tmax = fminf(tmax, fmaxf(ty1, ty2));
+        const float tz1 = -A.z * rV.z, tz2 = (1024.0f - A.z) * rV.z;
+        tmin = fmaxf(tmin, fminf(tz1, tz2)), tmax = fminf(tmax, fmaxf(tz1, tz2));
+        if (tmax < tmin || tmax <= 0) active = false;
+        A.x += tmin * V.x, A.y += tmin * V.y, A.z += tmin * V.z;
+        to = tmin;
+        if (A.y < 0.01f || A.y > (1024.0f - 1.01f)) last = 1;
+        if (A.z < 0.01f || A.z > (1024.0f - 1.01f)) last = 2;
+    }
+    uint32_t up = (min(f2u(A.x) >> 5, 31u) << 20) + (min(f2u(A.y) >> 5, 31u) << 10) + min(f2u(A.z) >> 5, 31u);
+    // tm of the uber level (u*), of the top grid (g*) and of the brick (b*)
+    float ux = ((float)((up >> 20) + ox) - A.x * 0.03125f) * rV.x;
+    float uy = ((float)(((up >> 10) & 31u) + oy) - A.y * 0.03125f) * rV.y;
+    float uz = ((float)((up & 31u) + oz) - A.z * 0.03125f) * rV.z;
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+    float t = 0.0f;
+    uint32_t tp = 0u, p = 0u, brick = 0u;
+    // the lane's level: 0 uber, 1 top grid, 2 brick; 3 done without a hit,
+    // 4 hit a solid top-grid cell, 5 hit a voxel.  `resume` (see above) for
+    // the level it is at.  Inside a unit the lanes of that level update their
+    // state with selects (the other lanes keep theirs), so the loop has
+    // little control flow for the compiler to copy registers across.
+    uint32_t lv = active ? 0u : 3u, resume = 0u;
+    // the occupancy of the lane's current cell: uber bit, top-grid word or voxel
+    uint32_t o = active ? uber_bit(w.uber, (up >> 20) + ((up & 31u) << 5) + (((up >> 10) & 31u) << 10)) : 0u;
+    while (true) {
+        const uint32_t n0 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 0u));
+        const uint32_t n1 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 1u));
+        const uint32_t n2 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 2u));
+        if ((n0 | n1 | n2) == 0u) break;
+        if (n2 >= n1 && n2 >= n0) {
+            // a voxel of the brick walk
+            const bool mine = lv == 2u;
+            const bool stop = mine && (o & 0xf000u) == 0xf000u;
+            const bool go = mine && !stop;
+            const float tn = fminf(bx, fminf(by, bz));
+            const bool sx = go && tn == bx, sy = go && tn == by, sz = go && tn == bz;
+            bx = sx ? bx + td.x : bx;
+            by = sy ? by + td.y : by;
+            bz = sz ? bz + td.z : bz;
+            p += (sx ? dx : 0u) + (sy ? dy : 0u) + (sz ? dz : 0u);
+            last = go ? (sz ? 2u : (sy ? 1u : 0u)) : last;
+            t = go ? tn : t;
+            const bool out = go && (p & kTopMask3) != 0u;
+            lv = stop ? 5u : (out ? 1u : lv);
+            resume = out ? 1u : resume;
+            if (go && !out) o = w.bricks[(size_t)brick * 512u + (p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
+        } else if (n1 >= n0) {
+            // a cell of the top grid: visit (budget, solid cell, brick), then
+            // step unless the lane stopped or went into a brick
+            const bool mine = lv == 1u;
+            const bool visit = mine && resume == 0u;
+            steps = visit ? steps - 1 : steps;
+            const bool spent = visit && steps == 0;
+            const bool solid = visit && !spent && (o & 1u) == 0u;
+            const bool stop = solid && ((o >> 1) & 0xf000u) == 0xf000u;
+            const bool down = visit && !spent && (o & 1u) != 0u;
+            if (down) {
+                t *= 8.0f;
+                const uint32_t q4x = f2u(A.x + V.x * t);
+                const uint32_t q4y = f2u(A.y + V.y * t);
+                const uint32_t q4z = f2u(A.z + V.z * t);
+                const uint32_t cx8 = (tp >> 20) << 3, cy8 = ((tp >> 10) & 127u) << 3, cz8 = (tp & 127u) << 3;
+                p = (clampu(q4x, cx8, cx8 + 7) << 20) + (clampu(q4y, cy8, cy8 + 7) << 10) + clampu(q4z, cz8, cz8 + 7);
+                bx = ((float)((p >> 20) + ox) - A.x) * rV.x;
+                by = ((float)(((p >> 10) & 1023u) + oy) - A.y) * rV.y;
+                bz = ((float)((p & 1023u) + oz) - A.z) * rV.z;
+                p &= 7u + (7u << 10) + (7u << 20);
+                brick = o >> 1;
+                o = w.bricks[(size_t)brick * 512u + (p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
+            }
+            const bool go = mine && !spent && !stop && !down;
+            const float tn = fminf(gx, fminf(gy, gz));
+            const bool sx = go && tn == gx, sy = go && tn == gy, sz = go && tn == gz;
+            gx = sx ? gx + td.x : gx;
+            gy = sy ? gy + td.y : gy;
+            gz = sz ? gz + td.z : gz;
+            tp += (sx ? dx : 0u) + (sy ? dy : 0u) + (sz ? dz : 0u);
+            last = go ? (sz ? 2u : (sy ? 1u : 0u)) : last;
+            t = go ? tn : t;
+            const bool out = go && (tp & kUberMask3) != (up << 2);
+            lv = stop ? 4u : (down ? 2u : ((spent || out) ? 0u : lv));
+            resume = (spent || out) ? 1u : (mine ? 0u : resume);
+            if (go && !out) o = w.grid[(tp >> 20) + ((tp & 127u) << 7) + (((tp >> 10) & 127u) << 14)];
+        } else {
+            // a cell of the uber grid: visit (budget, occupied cell), then
+            // step unless the lane stopped or went into the top grid
+            const bool mine = lv == 0u;
+            const bool visit = mine && resume == 0u;
+            steps = visit ? steps - 4 : steps;
+            const bool spent = visit && steps <= 0;
+            const bool down = visit && !spent && o != 0u;
+            if (down) {
+                t *= 4.0f;
+                const uint32_t p4x = f2u(0.125f * A.x + V.x * t);
+                const uint32_t p4y = f2u(0.125f * A.y + V.y * t);
+                const uint32_t p4z = f2u(0.125f * A.z + V.z * t);
+                const uint32_t bx4 = (up >> 20) << 2, by4 = ((up >> 10) & 31u) << 2, bz4 = (up & 31u) << 2;
+                tp = (clampu(p4x, bx4, bx4 + 3) << 20) + (clampu(p4y, by4, by4 + 3) << 10) + clampu(p4z, bz4, bz4 + 3);
+                gx = ((float)((tp >> 20) + ox) - A.x * 0.125f) * rV.x;
+                gy = ((float)(((tp >> 10) & 127u) + oy) - A.y * 0.125f) * rV.y;
+                gz = ((float)((tp & 127u) + oz) - A.z * 0.125f) * rV.z;
+                o = w.grid[(tp >> 20) + ((tp & 127u) << 7) + (((tp >> 10) & 127u) << 14)];
+            }
+            const bool go = mine && !spent && !down;
+            const float tn = fminf(ux, fminf(uy, uz));
+            const bool sx = go && tn == ux, sy = go && tn == uy, sz = go && tn == uz;
+            ux = sx ? ux + td.x : ux;
+            uy = sy ? uy + td.y : uy;
+            uz = sz ? uz + td.z : uz;
+            up += (sx ? dx : 0u) + (sy ? dy : 0u) + (sz ? dz : 0u);
+            last = go ? (sz ? 2u : (sy ? 1u : 0u)) : last;
+            t = go ? tn : t;
+            const bool out = go && (up & 0xfe0f83e0u) != 0u;
+            lv = (spent || out) ? 3u : (down ? 1u : lv);
+            resume = mine ? 0u : resume;
+            if (go && !out) o = uber_bit(w.uber, (up >> 20) + ((up & 31u) << 5) + (((up >> 10) & 31u) << 10));
+        }
+    }
+    uint32_t hit = 0u;
+    if (lv == 5u) hit = o, dist = t + to, side = last;
+    if (lv == 4u) hit = o >> 1, dist = t * 8.0f + to, side = last;
+    return hit;
+}
+
 // ---------------------------------------------------------------------------
 // camera ray (cl/tools.cl:102-127, TAA=0, PANINI=0)
 // ---------------------------------------------------------------------------
