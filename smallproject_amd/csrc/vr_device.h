@@ -374,22 +374,28 @@ __device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float
         const uint32_t n2 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 2u));
         if ((n0 | n1 | n2) == 0u) break;
         if (n2 >= n1 && n2 >= n0) {
-            // a voxel of the brick walk
-            const bool mine = lv == 2u;
-            const bool stop = mine && (o & 0xf000u) == 0xf000u;
-            const bool go = mine && !stop;
-            const float tn = fminf(bx, fminf(by, bz));
-            const bool sx = go && tn == bx, sy = go && tn == by, sz = go && tn == bz;
-            bx = sx ? bx + td.x : bx;
-            by = sy ? by + td.y : by;
-            bz = sz ? bz + td.z : bz;
-            p += (sx ? dx : 0u) + (sy ? dy : 0u) + (sz ? dz : 0u);
-            last = go ? (sz ? 2u : (sy ? 1u : 0u)) : last;
-            t = go ? tn : t;
-            const bool out = go && (p & kTopMask3) != 0u;
-            lv = stop ? 5u : (out ? 1u : lv);
-            resume = out ? 1u : resume;
-            if (go && !out) o = w.bricks[(size_t)brick * 512u + (p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
+            // voxels of the brick walks, for as long as most lanes are in a
+            // brick (an inner loop: its state stays in place between turns)
+            uint32_t m1 = n1, m2 = n2;
+            do {
+                const bool mine = lv == 2u;
+                const bool stop = mine && (o & 0xf000u) == 0xf000u;
+                const bool go = mine && !stop;
+                const float tn = fminf(bx, fminf(by, bz));
+                const bool sx = go && tn == bx, sy = go && tn == by, sz = go && tn == bz;
+                bx = sx ? bx + td.x : bx;
+                by = sy ? by + td.y : by;
+                bz = sz ? bz + td.z : bz;
+                p += (sx ? dx : 0u) + (sy ? dy : 0u) + (sz ? dz : 0u);
+                last = go ? (sz ? 2u : (sy ? 1u : 0u)) : last;
+                t = go ? tn : t;
+                const bool out = go && (p & kTopMask3) != 0u;
+                lv = stop ? 5u : (out ? 1u : lv);
+                resume = out ? 1u : resume;
+                if (go && !out) o = w.bricks[(size_t)brick * 512u + (p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
+                m1 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 1u));
+                m2 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 2u));
+            } while (m2 >= m1 && m2 >= n0 && m2 != 0u);
         } else if (n1 >= n0) {
             // a cell of the top grid: visit (budget, solid cell, brick), then
             // step unless the lane stopped or went into a brick
