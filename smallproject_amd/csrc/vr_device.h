@@ -316,8 +316,9 @@ __device__ uint32_t trace(const WorldView& w, f3 A, f3 B, int steps, uint32_t th
 // are not inside a brick wait (on c2's shadow rays about 4 of 64 lanes take
 // each brick step).  Here the same traversal is cut into units - one visit
 // and step of an uber cell, of a top cell, or of a voxel - and each turn of
-// the wave's loop runs the unit of the level that most of the wave's lanes
-// are at; the others keep their state.  Every lane performs the operations
+// the wave's loop runs the unit of one level, chosen from how many of the
+// wave's lanes are at each (ballot counts, so the choice is a scalar
+// branch); the other lanes keep their state.  Every lane performs the operations
 // of trace() in the same order (each level keeps its own tm registers, which
 // trace() saves and restores around a descent), so the results are the same
 // bits.  `resume`: the lane's next unit at its level is the step that
@@ -373,9 +374,15 @@ __device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float
         const uint32_t n1 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 1u));
         const uint32_t n2 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 2u));
         if ((n0 | n1 | n2) == 0u) break;
-        if (n2 >= n1 && n2 >= n0) {
-            // voxels of the brick walks, for as long as most lanes are in a
-            // brick (an inner loop: its state stays in place between turns)
+        // The level to run: bricks when at least half as many lanes are in
+        // a brick as at either other level, then the top grid when at least
+        // twice as many lanes are there as at the uber level, else the uber
+        // level (of the weightings tried on c2, the fastest: brick units are
+        // the cheapest, and running the uber level first gathers lanes into
+        // the top grid's more costly units)
+        if (2u * n2 >= n1 && 2u * n2 >= n0) {
+            // voxels of the brick walks, for as long as that holds (an inner
+            // loop: its state stays in place between turns)
             uint32_t m1 = n1, m2 = n2;
             do {
                 const bool mine = lv == 2u;
@@ -395,8 +402,8 @@ __device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float
                 if (go && !out) o = w.bricks[(size_t)brick * 512u + (p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
                 m1 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 1u));
                 m2 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 2u));
-            } while (m2 >= m1 && m2 >= n0 && m2 != 0u);
-        } else if (n1 >= n0) {
+            } while (2u * m2 >= m1 && 2u * m2 >= n0 && m2 != 0u);
+        } else if (n1 >= 2u * n0) {
             // a cell of the top grid: visit (budget, solid cell, brick), then
             // step unless the lane stopped or went into a brick
             const bool mine = lv == 1u;
