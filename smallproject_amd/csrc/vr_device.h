@@ -310,6 +310,14 @@ __device__ uint32_t trace(const WorldView& w, f3 A, f3 B, int steps, uint32_t th
     return 0u;
 }
 
+// voxel lx + 8 ly + 64 lz of a brick from a packed in-brick position
+// (lx << 20 | ly << 10 | lz, each 0..7): the three fields land on disjoint
+// bits, so a field extract and two shift-ors build the index
+__device__ __forceinline__ uint32_t brick_voxel(uint32_t p)
+{
+    return (p >> 20) | (__builtin_amdgcn_ubfe(p, 10u, 3u) << 3) | ((p & 7u) << 6);
+}
+
 // TraceOpaqueRay (trace.cl:205-282) for rays that go their own ways, as
 // shadow rays do.  trace() nests the brick walk inside the top-grid loop
 // inside the uber loop, so a wave runs each brick walk while the lanes that
@@ -399,7 +407,7 @@ __device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float
                 const bool out = go && (p & kTopMask3) != 0u;
                 lv = stop ? 5u : (out ? 1u : lv);
                 resume = out ? 1u : resume;
-                if (go && !out) o = w.bricks[(size_t)brick * 512u + (p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
+                if (go && !out) o = w.bricks[(size_t)brick * 512u + brick_voxel(p)];
                 m1 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 1u));
                 m2 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 2u));
             } while (2u * m2 >= m1 && 2u * m2 >= n0 && m2 != 0u);
@@ -425,7 +433,7 @@ __device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float
                 bz = ((float)((p & 1023u) + oz) - A.z) * rV.z;
                 p &= 7u + (7u << 10) + (7u << 20);
                 brick = o >> 1;
-                o = w.bricks[(size_t)brick * 512u + (p >> 20) + ((p >> 7) & 56u) + ((p & 7u) << 6)];
+                o = w.bricks[(size_t)brick * 512u + brick_voxel(p)];
             }
             const bool go = mine && !spent && !stop && !down;
             const float tn = fminf(gx, fminf(gy, gz));
