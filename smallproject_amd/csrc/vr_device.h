@@ -331,9 +331,14 @@ __device__ __forceinline__ uint32_t brick_voxel(uint32_t p)
 // trace() saves and restores around a descent), so the results are the same
 // bits.  `resume`: the lane's next unit at its level is the step that
 // follows a child level's exit (trace() steps the parent right after the
-// child loop ends).  A brick walk needs at most 22 steps (7 per axis and
-// the one that leaves), so trace()'s 24-step cap never ends one.
-__device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float& dist, uint32_t& side)
+// child loop ends).  While its steps advance a coordinate, a brick walk
+// leaves its brick within 22 steps (7 per axis and the one that leaves), so
+// trace()'s cap of 24 steps per brick never ends it; a ray whose steps do
+// not advance (a NaN direction: a shadow ray gets one only when its sample
+// point is the shading point itself) is ended by that cap and the step
+// budget, and kCap keeps the cap (see trace_sched below).
+template <bool kCap>
+__device__ uint32_t trace_sched_walk(const WorldView& w, f3 A, f3 B, int steps, float& dist, uint32_t& side)
 {
     if (fabsf(B.x) < 1e-8f) B.x = B.x < 0 ? -1e-8f : 1e-8f;
     if (fabsf(B.y) < 1e-8f) B.y = B.y < 0 ? -1e-8f : 1e-8f;
@@ -404,9 +409,9 @@ __device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float
                 p += (sx ? dx : 0u) + (sy ? dy : 0u) + (sz ? dz : 0u);
                 last = go ? (sz ? 2u : (sy ? 1u : 0u)) : last;
                 t = go ? tn : t;
-                const bool out = go && (p & kTopMask3) != 0u;
+                const bool out = go && ((p & kTopMask3) != 0u || (kCap && resume == 23u));
                 lv = stop ? 5u : (out ? 1u : lv);
-                resume = out ? 1u : resume;
+                resume = out ? 1u : (kCap && go ? resume + 1u : resume);
                 if (go && !out) o = w.bricks[(size_t)brick * 512u + brick_voxel(p)];
                 m1 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 1u));
                 m2 = (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(lv == 2u));
@@ -487,6 +492,22 @@ __device__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float
     if (lv == 5u) hit = o, dist = t + to, side = last;
     if (lv == 4u) hit = o >> 1, dist = t * 8.0f + to, side = last;
     return hit;
+}
+
+// With every coordinate of the origin and the direction within +-2^20 (so
+// finite: a NaN fails the comparisons), 2^-27 < |1/V| < 2^27 after
+// FixZeroDeltas, and every tm, t and position product of the walk stays
+// finite, so each step advances a coordinate.  A wave with a ray outside
+// those bounds (a NaN direction; vr_trace_rays refuses non-finite rays but
+// takes large ones) walks with trace()'s cap of 24 steps per brick, counted
+// in `resume`; other waves do not pay for the count.
+__device__ __forceinline__ uint32_t trace_sched(const WorldView& w, f3 A, f3 B, int steps, float& dist, uint32_t& side)
+{
+    constexpr float kBound = 1048576.0f;
+    const bool bounded = fabsf(A.x) <= kBound && fabsf(A.y) <= kBound && fabsf(A.z) <= kBound &&
+                         fabsf(B.x) <= kBound && fabsf(B.y) <= kBound && fabsf(B.z) <= kBound;
+    if (__builtin_amdgcn_ballot_w64(!bounded) != 0u) return trace_sched_walk<true>(w, A, B, steps, dist, side);
+    return trace_sched_walk<false>(w, A, B, steps, dist, side);
 }
 
 // ---------------------------------------------------------------------------

@@ -74,13 +74,19 @@ __device__ __forceinline__ void count_rays(unsigned long long* rays, int k, uint
 // GridCoordinatesFromHit + Uint3CubeEquals (cl/tools.cl:452-472) behind
 // isLightOccluded (cl/ris.cl:1-17).  A shadow ray that finds no opaque voxel
 // within its step budget counts as occluded.  Shadow rays of a wave head to
-// different lights, so they take the wave-scheduled DDA.
+// different lights, so they take the wave-scheduled DDA: kPick picks the
+// walk per wave (trace_sched: uncounted unless a ray is out of bounds), else
+// the walk that counts its brick steps runs always (one copy of the loop:
+// in the spatial reuse + shading kernel a second copy costs more registers
+// than the count costs its fewer rays).
+template <bool kPick>
 __device__ __forceinline__ bool light_occluded(const WorldView& w, uint32_t lpos, uint32_t lsize, f3 spo, f3 L,
                                                int steps)
 {
     float dist = 0.0f;
     uint32_t side = 0;
-    const uint32_t v = trace_sched(w, spo, L, steps, dist, side);
+    const uint32_t v = kPick ? trace_sched(w, spo, L, steps, dist, side)
+                             : trace_sched_walk<true>(w, spo, L, steps, dist, side);
     if (v == 0u) return true;
     const f3 N = voxel_normal(side, L);
     const uint32_t hx = f2u((L.x * dist + spo.x) - N.x * 0.1f);
@@ -425,7 +431,7 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
             const float4 lb = a.L.b[res.light];
             const float4 la = a.L.a[res.light];
             const f3 L = normalize(sub(res.pos, spo));
-            occ = light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
+            occ = light_occluded<true>(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
             shadow = kTraced | (occ ? 0u : kVisible);
             traced = 1u;
         }
@@ -676,7 +682,7 @@ __device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int 
             const f3 spo = f3{sh.sp.x + 0.1f * sh.N.x, sh.sp.y + 0.1f * sh.N.y, sh.sp.z + 0.1f * sh.N.z};
             const f3 Ld = normalize(sub(f3{smp.x, smp.y, smp.z}, spo));
             const float4 tla = a.L.a[light], tlb = a.L.b[light];
-            if (light_occluded(a.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, a.shadowSteps))
+            if (light_occluded<false>(a.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, a.shadowSteps))
                 atomicOr(&L.occluded[owner], 1u << (tk >> 8));
         }
         __syncthreads();
@@ -763,7 +769,7 @@ __device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
                     if (kDefer && known < 0) return true;
                     const bool visible =
                         known >= 0 ? known == 1
-                                   : !light_occluded(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
+                                   : !light_occluded<false>(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
                     // what the traversal did not keep in registers: from LDS
                     // when parked, else the values themselves
                     const float4 pp = park ? vload(pk.p[t]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);

@@ -60,6 +60,53 @@ def test_dda_matches_oracle(gpu, name):
                     assert hit.mean() > 0.05
 
 
+def test_dda_degenerate_directions(gpu):
+    """Zero and denormal directions (FixZeroDeltas turns every component
+    into +-1e-8, cl/trace.cl:37-43) through TraceRay and TraceOpaqueRay,
+    with the oracle's hits.  vr_trace_rays refuses non-finite rays: a NaN
+    direction's steps never advance, and such a ray would walk its whole
+    step budget (the DDA still ends it there, and a brick walk after 24
+    steps, cl/trace.cl:50-111, as the shadow rays' scheduled DDA does)."""
+    sc = configs.CONFIGS["c2_area_1024"]
+    w = configs.build_world(sc, 64, 64)
+    w.force_sync()
+    view = O.View(w.grid(), w.bricks())
+    rng = np.random.default_rng(5)
+    Orig, D = _rays(rng, 4096, 0.0, float(sc.world_size))
+    D[::3] = 0.0
+    D[1::7] = np.float32(1e-39)
+    D[2::11, 1:] = 0.0
+    for kind in (_lib.TRACE_ALL, _lib.TRACE_OPAQUE):
+        for steps in (128, 40):
+            gv, gd, gs = w.renderer.trace(Orig, D, kind, steps)
+            ov, od, os_ = O.trace_batch(view, Orig, D, steps, kind)
+            hit = ov != 0
+            assert np.array_equal(gv, ov), f"kind {kind} steps {steps}: voxel differs on {(gv != ov).sum()} rays"
+            assert np.array_equal(gs[hit], os_[hit])
+            assert np.array_equal(gd[hit].view(np.uint32), od[hit].view(np.uint32))
+            assert hit.mean() > 0.1
+    # origins millions of voxels out (beyond the scheduled DDA's +-2^20
+    # bound, so these waves walk with the 24-step cap counted) aimed at the map
+    far = rng.uniform(-4e6, 4e6, size=(4096, 3)).astype(np.float32)
+    aim = (rng.uniform(0, sc.world_size, size=(4096, 3)).astype(np.float32) - far).astype(np.float32)
+    aim /= np.linalg.norm(aim, axis=1, keepdims=True)
+    for kind in (_lib.TRACE_ALL, _lib.TRACE_OPAQUE):
+        for steps in (999999, 128):
+            gv, gd, gs = w.renderer.trace(far, aim, kind, steps)
+            ov, od, os_ = O.trace_batch(view, far, aim, steps, kind)
+            hit = ov != 0
+            assert np.array_equal(gv, ov), f"far origins kind {kind} steps {steps}: voxel differs on {(gv != ov).sum()} rays"
+            assert np.array_equal(gs[hit], os_[hit])
+            assert np.array_equal(gd[hit].view(np.uint32), od[hit].view(np.uint32))
+    bad = D.copy()
+    bad[7] = np.nan
+    with pytest.raises(_lib.VrError):
+        w.renderer.trace(Orig, bad, _lib.TRACE_OPAQUE, 128)
+    bad[7] = (np.inf, 0.0, 0.0)
+    with pytest.raises(_lib.VrError):
+        w.renderer.trace(Orig, bad, _lib.TRACE_OPAQUE, 128)
+
+
 def test_c1_point_lights_8_frames(gpu):
     """Configuration 1 (640x360, point lights, 64 emitters, 8 frames)."""
     res = run_config(configs.CONFIGS["c1_point_128"], frames=8)
