@@ -145,15 +145,19 @@ __device__ __forceinline__ void point_on_voxel(f3 c, float size, f3 sp, uint32_t
         const float spa = a == 0 ? sp.x : (a == 1 ? sp.y : sp.z);
         plusA[a] = (ca + hs) - spa < 0.0f;
     }
-    float q0[3], q1[3];
+    // the pair of each axis is picked as integers and converted once
+    uint32_t w0[3], w1[3];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         const uint32_t v0 = xorshift(seed), v1 = xorshift(seed);
         const int a = k % 3;
-        if (plusA[a] == (k < 3)) {
-            q0[a] = (float)v0 * 2.3283064365387e-10f;
-            q1[a] = (float)v1 * 2.3283064365387e-10f;
-        }
+        if (plusA[a] == (k < 3)) w0[a] = v0, w1[a] = v1;
+    }
+    float q0[3], q1[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        q0[a] = (float)w0[a] * 2.3283064365387e-10f;
+        q1[a] = (float)w1[a] * 2.3283064365387e-10f;
     }
     const float r2 = rnd(seed);
     float sol[3];
