@@ -347,8 +347,8 @@ def main() -> int:
         step(f)
     r.sync()
     log(f"bench: {args.warmup} warm-up frames done")
-    from smallproject_amd.autotune import ORDERS, confirm, tune_tile_order, tune_waves
-    waves, waves_ms, confirm_ms = None, None, None
+    from smallproject_amd.autotune import ORDERS, confirm, tune_dda, tune_tile_order, tune_waves
+    waves, waves_ms, dda, dda_ms, confirm_ms = None, None, None, None, None
     if args.tile_order == "auto":
         tile_order, tune_ms = tune_tile_order(r, lambda i: step(args.warmup + i))
         log("bench: tile orders " + "; ".join(f"{k} " + " ".join(f"{v:.4f}" for v in ms) for k, ms in tune_ms.items())
@@ -356,12 +356,16 @@ def main() -> int:
         waves, waves_ms = tune_waves(r, lambda i: step(args.warmup + i))
         log("bench: waves per SIMD " + "; ".join(f"{k} " + " ".join(f"{v:.4f}" for v in ms)
                                                 for k, ms in waves_ms.items()) + f" -> {waves}")
-        kept, confirm_ms = confirm(r, lambda i: step(args.warmup + i), tile_order, waves)
+        dda, dda_ms = tune_dda(r, lambda i: step(args.warmup + i))
+        log("bench: shadow-ray traversal " + "; ".join(f"{k} " + " ".join(f"{v:.4f}" for v in ms)
+                                                    for k, ms in dda_ms.items()) + f" -> {dda}")
+        kept, confirm_ms = confirm(r, lambda i: step(args.warmup + i), tile_order, waves, dda)
         log(f"bench: whole frames, tuned {confirm_ms['tuned']:.4f} ms, defaults {confirm_ms['default']:.4f} ms -> "
             + ("tuned" if kept else "defaults"))
         if not kept:
             tile_order = {name: "linear" for name in tile_order}
             waves = {name: 0 for name in waves}
+            dda = {name: "scheduled" for name in dda}
     else:
         tile_order, tune_ms = args.tile_order, None
         r.set_tile_order(ORDERS[tile_order])
@@ -441,6 +445,9 @@ def main() -> int:
                    "waves_per_simd": waves,
                    "waves_stage_ms": None if waves_ms is None else
                    {("default" if k == 0 else str(k)): [round(v, 4) for v in ms] for k, ms in waves_ms.items()},
+                   "shadow_dda": dda,
+                   "shadow_dda_stage_ms": None if dda_ms is None else
+                   {k: [round(v, 4) for v in ms] for k, ms in dda_ms.items()},
                    "tuned_vs_default_frame_ms": None if confirm_ms is None else
                    {k: round(v, 4) for k, v in confirm_ms.items()}},
         "mrays_per_s": round(fps * npx * RAYS_PER_PIXEL / 1e6, 1),

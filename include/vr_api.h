@@ -229,6 +229,17 @@ int vr_set_tile_order(vr_ctx* ctx, int32_t stage, uint32_t order);
  * More waves hide more of the DDA's load latency; fewer spill less. */
 int vr_set_stage_waves(vr_ctx* ctx, int32_t stage, uint32_t waves);
 
+/* Traversal of the shadow rays of stage VR_STAGE_INITIAL or
+ * VR_STAGE_RESOLVE (-1: both; the primary stage's rays always take the
+ * nested DDA) for the following frames: VR_DDA_SCHEDULED (the default) runs
+ * each turn of a wave's loop at the DDA level most of its lanes are at,
+ * VR_DDA_NESTED walks the uber, top-grid and brick levels as nested loops,
+ * which is faster where a wave's rays stay coherent.  A performance setting
+ * with no counterpart in the reference (outputs are identical). */
+#define VR_DDA_SCHEDULED 0u
+#define VR_DDA_NESTED    1u
+int vr_set_stage_dda(vr_ctx* ctx, int32_t stage, uint32_t mode);
+
 /* Read-backs (blocking), n = width*height.
  * hits (current frame): hitword = voxel | side<<24 | (D[side] > 0)<<26,
  * dist, seed.  Any output pointer may be NULL. */

@@ -23,6 +23,7 @@ QUIRK_DOUBLE_RADIANCE, QUIRK_AXIS_ONLY_FACE, QUIRK_SHADOW_STEP_CAP = 1, 2, 4
 QUIRKS_REFERENCE = 7
 TRACE_ALL, TRACE_OPAQUE = 0, 1
 TILES_LINEAR, TILES_XCD_ROWS, TILES_XCD_COLS, TILES_XCD_BLOCKS = 0, 1, 2, 3
+DDA_SCHEDULED, DDA_NESTED = 0, 1
 FLAG_MIS_SPATIAL, MIS_MAX_TAPS = 1 << 8, 32
 MAX_VIEWS = 64
 NOALLOC_MAX_RAYS = 1 << 18  # VR_NOALLOC_MAX_RAYS
@@ -73,6 +74,7 @@ SIGNATURES = {
     "vr_sync": (C.c_int, [_P]),
     "vr_set_tile_order": (C.c_int, [_P, _i32, _u32]),
     "vr_set_stage_waves": (C.c_int, [_P, _i32, _u32]),
+    "vr_set_stage_dda": (C.c_int, [_P, _i32, _u32]),
     "vr_read_hits": (C.c_int, [_P, _P, _P, _P]),
     "vr_read_reservoirs": (C.c_int, [_P, C.c_int, _P, _P, _P]),
     "vr_read_frame": (C.c_int, [_P, _P]),

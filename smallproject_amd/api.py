@@ -90,6 +90,11 @@ class Renderer:
         """Occupancy target (5-8 waves per SIMD, 0 = default) of a stage (-1: all)."""
         self._chk(lib().vr_set_stage_waves(self.h, stage, waves), "vr_set_stage_waves")
 
+    def set_stage_dda(self, mode: int, stage: int = -1):
+        """Shadow-ray traversal (DDA_SCHEDULED or DDA_NESTED) of a stage (-1:
+        the candidate and resolve stages; the primary stage has one)."""
+        self._chk(lib().vr_set_stage_dda(self.h, stage, mode), "vr_set_stage_dda")
+
     def set_tile_order(self, order: int, stage: int = -1):
         self._chk(lib().vr_set_tile_order(self.h, stage, order), "vr_set_tile_order")
 

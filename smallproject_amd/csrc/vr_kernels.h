@@ -51,6 +51,7 @@ struct FrameArgs {
     uint32_t tileOrder;  // VR_TILES_*
     uint32_t misSpatial; // spatial merge with MIS weights (VR_FLAG_MIS_SPATIAL)
     uint32_t waves;      // occupancy target of this stage's kernel (0: kDefaultWaves)
+    uint32_t dda;        // traversal of this stage's shadow rays (kDda*)
     // learned source: candidate slot j of pixel i at [j * W*H + i] holds a
     // cluster and the bits of its mixture mass; inside the cluster the light
     // is drawn with the cluster's power alias table
@@ -112,6 +113,11 @@ __device__ __forceinline__ void tile_of_block(uint32_t order, int& tx, int& ty)
     tx = (int)(t % nbx);
     ty = (int)(t / nbx);
 }
+
+// traversal of a stage's shadow rays (FrameArgs::dda; the values of
+// VR_DDA_SCHEDULED and VR_DDA_NESTED): the wave-scheduled DDA, or the
+// nested one of the primary rays
+constexpr uint32_t kDdaScheduled = 0u, kDdaNested = 1u;
 
 // default occupancy targets (waves per SIMD) of the stage kernels: primary,
 // candidates, spatial reuse + shading.  The primary pass mostly runs beside

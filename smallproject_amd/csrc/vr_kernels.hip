@@ -78,15 +78,17 @@ __device__ __forceinline__ void count_rays(unsigned long long* rays, int k, uint
 // walk per wave (trace_sched: uncounted unless a ray is out of bounds), else
 // the walk that counts its brick steps runs always (one copy of the loop:
 // in the spatial reuse + shading kernel a second copy costs more registers
-// than the count costs its fewer rays).
-template <bool kPick>
+// than the count costs its fewer rays).  kNested: the nested DDA instead
+// (a stage's choice, kDdaNested: scenes whose rays stay coherent).
+template <bool kPick, bool kNested>
 __device__ __forceinline__ bool light_occluded(const WorldView& w, uint32_t lpos, uint32_t lsize, f3 spo, f3 L,
                                                int steps)
 {
     float dist = 0.0f;
     uint32_t side = 0;
-    const uint32_t v = kPick ? trace_sched(w, spo, L, steps, dist, side)
-                             : trace_sched_walk<true>(w, spo, L, steps, dist, side);
+    const uint32_t v = kNested ? trace<kTraceOpaque>(w, spo, L, steps, 0u, dist, side)
+                               : (kPick ? trace_sched(w, spo, L, steps, dist, side)
+                                        : trace_sched_walk<true>(w, spo, L, steps, dist, side));
     if (v == 0u) return true;
     const f3 N = voxel_normal(side, L);
     const uint32_t hx = f2u((L.x * dist + spo.x) - N.x * 0.1f);
@@ -359,6 +361,7 @@ __device__ __forceinline__ float4 vload(const float4& p)
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+template <bool kNested>
 __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
 {
     const size_t idx = (size_t)y * a.W + x;
@@ -435,7 +438,7 @@ __device__ __forceinline__ void initial_px(const FrameArgs& a, int x, int y)
             const float4 lb = a.L.b[res.light];
             const float4 la = a.L.a[res.light];
             const f3 L = normalize(sub(res.pos, spo));
-            occ = light_occluded<true>(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
+            occ = light_occluded<true, kNested>(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
             shadow = kTraced | (occ ? 0u : kVisible);
             traced = 1u;
         }
@@ -614,6 +617,7 @@ struct MisLds {
     uint32_t count[2];                // list length, per chunk parity
 };
 
+template <bool kNested>
 __device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int y, const Merge& g, bool in)
 {
     __shared__ MisLds L;
@@ -686,7 +690,7 @@ __device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int 
             const f3 spo = f3{sh.sp.x + 0.1f * sh.N.x, sh.sp.y + 0.1f * sh.N.y, sh.sp.z + 0.1f * sh.N.z};
             const f3 Ld = normalize(sub(f3{smp.x, smp.y, smp.z}, spo));
             const float4 tla = a.L.a[light], tlb = a.L.b[light];
-            if (light_occluded<false>(a.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, a.shadowSteps))
+            if (light_occluded<false, kNested>(a.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, a.shadowSteps))
                 atomicOr(&L.occluded[owner], 1u << (tk >> 8));
         }
         __syncthreads();
@@ -728,7 +732,7 @@ __device__ __forceinline__ ShadeParkLds& shade_park_lds()
 
 // kPark: the call that traces (kDefer false) parks the live values in LDS
 // across the traversal (the MIS merge's kernel has no LDS to spare for it)
-template <bool kDefer = false, bool kPark = false>
+template <bool kDefer = false, bool kPark = false, bool kNested = false>
 __device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
 {
     constexpr bool park = kPark && !kDefer;
@@ -773,7 +777,7 @@ __device__ __forceinline__ bool shade_px(const FrameArgs& a, int x, int y)
                     if (kDefer && known < 0) return true;
                     const bool visible =
                         known >= 0 ? known == 1
-                                   : !light_occluded<false>(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
+                                   : !light_occluded<false, kNested>(a.w, __float_as_uint(lb.w), (uint32_t)la.w, spo, L, a.shadowSteps);
                     // what the traversal did not keep in registers: from LDS
                     // when parked, else the values themselves
                     const float4 pp = park ? vload(pk.p[t]) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -910,10 +914,10 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_primary(FrameArgs a)
 {
     one_view(a, [](const FrameArgs& v, int x, int y) { primary_px(v, x, y); });
 }
-template <int W>
+template <int W, bool kNested>
 __global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
 {
-    one_view(a, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
+    one_view(a, [](const FrameArgs& v, int x, int y) { initial_px<kNested>(v, x, y); });
 }
 // Spatial reuse, then shading of the tile's 256 pixels.  Only the pixels
 // whose final sample needs a shadow ray the candidate pass did not trace
@@ -924,7 +928,7 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
 // thread reaches every barrier (the MIS weight's included).
 // kPark: shading parks its live values in LDS across the traversal (not
 // where a kernel also holds the MIS merge's LDS)
-template <bool kMis, bool kPark = !kMis>
+template <bool kMis, bool kNested, bool kPark = !kMis>
 __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
 {
     __shared__ uint32_t rayPx[256];  // threads of the tile whose pixel needs a ray
@@ -933,7 +937,7 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
     if (kMis) {
         Merge g{};
         if (in) g = spatial_merge<true>(a, x, y);
-        const float W = mis_weight_tile(a, x, y, g, in);
+        const float W = mis_weight_tile<kNested>(a, x, y, g, in);
         if (in) {
             const size_t idx = (size_t)y * a.W + x;
             if (g.merged) {
@@ -945,7 +949,7 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
     } else if (in) {
         spatial_px(a, x, y);
     }
-    const bool defer = in && shade_px<true, kPark>(a, x, y);
+    const bool defer = in && shade_px<true, kPark, kNested>(a, x, y);
     const uint64_t m = __ballot(defer);
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     if (lane == 0u) waveRays[wv] = (uint32_t)__popcll(m);
@@ -962,11 +966,11 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
         // the queued thread's pixel: same tile, its own wave quadrant and lane
         const uint32_t q = rayPx[threadIdx.x], qw = q >> 6, ql = q & 63u;
         const int x0 = x - (int)((wv & 1u) * 8u + (lane & 7u)), y0 = y - (int)((wv >> 1) * 8u + (lane >> 3));
-        shade_px<false, kPark>(a, x0 + (int)((qw & 1u) * 8u + (ql & 7u)), y0 + (int)((qw >> 1) * 8u + (ql >> 3)));
+        shade_px<false, kPark, kNested>(a, x0 + (int)((qw & 1u) * 8u + (ql & 7u)), y0 + (int)((qw >> 1) * 8u + (ql >> 3)));
     }
 }
 
-template <bool kMis, int W>
+template <bool kMis, int W, bool kNested>
 __global__ __launch_bounds__(256) VR_WAVES(W) void k_resolve(FrameArgs a)
 {
     stage_uber(a.w.uber);
@@ -974,7 +978,7 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_resolve(FrameArgs a)
     tile_pixel(a.tileOrder, x, y);
     FrameArgs b = a;
     b.w.uber = uber_lds();
-    resolve_tile<kMis>(b, x, y);
+    resolve_tile<kMis, kNested>(b, x, y);
 }
 
 template <bool kPersistent>
@@ -985,7 +989,7 @@ __global__ __launch_bounds__(256) VR_WAVES(8) void k_primary_views(ViewChunk c)
 template <bool kPersistent>
 __global__ __launch_bounds__(256) VR_WAVES(7) void k_initial_views(ViewChunk c)
 {
-    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { initial_px(v, x, y); });
+    many_views<kPersistent>(c, [](const FrameArgs& v, int x, int y) { initial_px<false>(v, x, y); });
 }
 // kAnyMis: some view merges with MIS weights (the kernel then carries the
 // MIS variant's registers).  Batch frames do not overlap one another, so the
@@ -994,8 +998,8 @@ template <bool kPersistent, bool kAnyMis>
 __global__ __launch_bounds__(256) VR_WAVES(kAnyMis ? kDefaultWavesMis : 7) void k_resolve_views(ViewChunk c)
 {
     many_views<kPersistent, true>(c, [](const FrameArgs& v, int x, int y) {
-        if (kAnyMis && v.misSpatial) resolve_tile<true>(v, x, y);
-        else resolve_tile<false, !kAnyMis>(v, x, y);
+        if (kAnyMis && v.misSpatial) resolve_tile<true, false>(v, x, y);
+        else resolve_tile<false, false, !kAnyMis>(v, x, y);
     });
 }
 
@@ -1118,33 +1122,42 @@ __global__ __launch_bounds__(256) void k_ground_truth(FrameArgs a, uint32_t spp,
 static inline dim3 frame_grid(uint32_t W, uint32_t H) { return dim3((W + 15) / 16, (H + 15) / 16); }
 
 template <int W>
-static const void* stage_kernel_w(int stage, bool mis)
+static const void* stage_kernel_w(int stage, bool mis, bool nested)
 {
     switch (stage) {
     case 0: return reinterpret_cast<const void*>(&k_primary<W>);
-    case 1: return reinterpret_cast<const void*>(&k_initial<W>);
-    case 2: return mis ? reinterpret_cast<const void*>(&k_resolve<true, W>)
-                       : reinterpret_cast<const void*>(&k_resolve<false, W>);
+    case 1: return nested ? reinterpret_cast<const void*>(&k_initial<W, true>)
+                          : reinterpret_cast<const void*>(&k_initial<W, false>);
+    case 2:
+        if (mis)
+            return nested ? reinterpret_cast<const void*>(&k_resolve<true, W, true>)
+                          : reinterpret_cast<const void*>(&k_resolve<true, W, false>);
+        return nested ? reinterpret_cast<const void*>(&k_resolve<false, W, true>)
+                      : reinterpret_cast<const void*>(&k_resolve<false, W, false>);
     default: return nullptr;
     }
 }
 
-// waves 0 = the stage's default target
-static const void* stage_kernel(int stage, bool mis, uint32_t waves)
+// waves 0 = the stage's default target; dda: kDda* of the stage's shadow
+// rays (the primary pass traces its coherent rays with the nested DDA
+// either way)
+static const void* stage_kernel(int stage, bool mis, uint32_t waves, uint32_t dda)
 {
     if (waves == 0) waves = stage == 2 && mis ? kDefaultWavesMis : kDefaultWaves[stage < 0 || stage > 2 ? 0 : stage];
+    if (dda > kDdaNested) return nullptr;
+    const bool nested = dda == kDdaNested;
     switch (waves) {
-    case 5: return stage_kernel_w<5>(stage, mis);
-    case 6: return stage_kernel_w<6>(stage, mis);
-    case 7: return stage_kernel_w<7>(stage, mis);
-    case 8: return stage_kernel_w<8>(stage, mis);
+    case 5: return stage_kernel_w<5>(stage, mis, nested);
+    case 6: return stage_kernel_w<6>(stage, mis, nested);
+    case 7: return stage_kernel_w<7>(stage, mis, nested);
+    case 8: return stage_kernel_w<8>(stage, mis, nested);
     default: return nullptr;
     }
 }
 
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s)
 {
-    const void* k = stage_kernel(stage, a.misSpatial != 0u, a.waves);
+    const void* k = stage_kernel(stage, a.misSpatial != 0u, a.waves, a.dda);
     if (!k) return hipErrorInvalidValue;
     void* args[] = {const_cast<FrameArgs*>(&a)};
     return hipLaunchKernel(k, frame_grid(a.W, a.H), dim3(256), args, 0, s);

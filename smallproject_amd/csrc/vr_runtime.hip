@@ -152,6 +152,7 @@ struct vr_ctx {
     uint32_t trainCap = 0;
     Camera lastCam{};             // camera of the last rendered frame
     uint32_t stageWaves[VR_NUM_STAGES] = {};  // occupancy target per stage (0: default)
+    uint32_t stageDda[VR_NUM_STAGES] = {};    // VR_DDA_* of each stage's shadow rays
     // scene motion on the device (allocated by vr_set_light_motion)
     float4* lightPos = nullptr;
     float4* lightVel = nullptr;
@@ -706,6 +707,7 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         FrameArgs as = a;
         as.tileOrder = (c->cfg.flags >> (2 * s)) & 3u;
         as.waves = c->stageWaves[s];
+        as.dda = c->stageDda[s];
         hipStream_t st = s == VR_STAGE_PRIMARY ? ps : c->stream;
         // the proposal runs after the primary hits and is timed with the
         // candidate stage: beside the previous frame's last stage too when
@@ -752,6 +754,19 @@ int vr_set_stage_waves(vr_ctx* c, int32_t stage, uint32_t waves)
     if (!c || stage < -1 || stage >= VR_NUM_STAGES || (waves != 0 && (waves < 5 || waves > 8))) return VR_ERR_ARG;
     for (int s = 0; s < VR_NUM_STAGES; s++)
         if (stage < 0 || stage == s) c->stageWaves[s] = waves;
+    return VR_OK;
+}
+
+static_assert(VR_DDA_SCHEDULED == kDdaScheduled && VR_DDA_NESTED == kDdaNested, "DDA mode values");
+
+int vr_set_stage_dda(vr_ctx* c, int32_t stage, uint32_t mode)
+{
+    if (!c) return VR_ERR_ARG;
+    if (mode > VR_DDA_NESTED) return fail(c, VR_ERR_ARG, "vr_set_stage_dda: unknown mode");
+    if (stage < -1 || stage == VR_STAGE_PRIMARY || stage >= VR_NUM_STAGES)
+        return fail(c, VR_ERR_ARG, "vr_set_stage_dda: unknown stage or the primary stage");
+    for (int s = VR_STAGE_INITIAL; s < VR_NUM_STAGES; s++)
+        if (stage < 0 || stage == s) c->stageDda[s] = mode;
     return VR_OK;
 }
 

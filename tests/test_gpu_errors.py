@@ -76,6 +76,9 @@ def _reject_everything(w):
     _rejects(_lib.VR_ERR_ARG, r.set_stage_waves, 4)
     _rejects(_lib.VR_ERR_ARG, r.set_stage_waves, 9)
     _rejects(_lib.VR_ERR_ARG, r.set_stage_waves, 6, _lib.NUM_STAGES)
+    _rejects(_lib.VR_ERR_ARG, r.set_stage_dda, _lib.DDA_NESTED + 1, match="unknown mode")
+    _rejects(_lib.VR_ERR_ARG, r.set_stage_dda, _lib.DDA_NESTED, 0, match="primary stage")
+    _rejects(_lib.VR_ERR_ARG, r.set_stage_dda, _lib.DDA_NESTED, _lib.NUM_STAGES, match="unknown stage")
     _rejects(_lib.VR_ERR_ARG, r.set_tile_order, _lib.TILES_XCD_BLOCKS + 1, match="unknown order")
     _rejects(_lib.VR_ERR_ARG, r.set_tile_order, _lib.TILES_LINEAR, _lib.NUM_STAGES, match="unknown stage")
 

@@ -276,6 +276,36 @@ def test_occupancy_targets_render_identical_frames(gpu):
         r2.set_stage_waves(9)
 
 
+@pytest.mark.parametrize("name,flags", [("c1_point_128", _lib.FLAG_MIS_SPATIAL), ("c1_translucent_128", 0),
+                                        ("c2_area_1024", _lib.FLAG_MIS_SPATIAL)])
+def test_shadow_traversals_render_identical_frames(gpu, name, flags):
+    """The scheduled and the nested shadow-ray DDA (vr_set_stage_dda), per
+    stage and together, compute the same bits: area lights with the MIS
+    merge so every stage traces, and translucent panes that shadow rays pass
+    through."""
+    import dataclasses
+    sc = dataclasses.replace(configs.CONFIGS[name], point=False, flags=flags)
+    N, S = _lib.DDA_NESTED, _lib.DDA_SCHEDULED
+    outs = []
+    for initial, resolve in ((S, S), (N, N), (N, S), (S, N)):
+        w = configs.build_world(sc, 160, 96)
+        w.force_sync()
+        r = w.renderer
+        r.set_stage_dda(initial, 1)
+        r.set_stage_dda(resolve, 2)
+        for f in range(3):
+            pos, tgt = sc.camera(f)
+            w.look_at(pos, tgt)
+            w.render()
+        hw, d, s = r.hits()
+        a, b, m = r.reservoirs(0)
+        outs.append((hw, d.view(np.uint32), s, a.view(np.uint32), b.view(np.uint32), m, r.frame().view(np.uint32)))
+        w.close()
+    for o in outs[1:]:
+        for k, (x, y) in enumerate(zip(outs[0], o)):
+            assert np.array_equal(x, y), f"output {k} differs between shadow-ray traversals"
+
+
 @pytest.mark.parametrize("name,flags", [("c2_area_1024", 0), ("c2_area_1024", _lib.FLAG_MIS_SPATIAL)])
 def test_back_to_back_frames_equal_synchronised_frames(gpu, name, flags):
     """Frames queued back to back let each frame's primary pass run on the
