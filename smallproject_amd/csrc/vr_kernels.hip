@@ -40,6 +40,21 @@ __device__ __forceinline__ Res res_from(const uint4 r0, const uint4 r1, const fl
 
 __device__ __forceinline__ Res load_res(const ResPlanes& r, size_t i) { return res_from(r.r0[i], r.r1[i], r.r2[i]); }
 
+// A neighbour's or the previous frame's record (r0) and sample (r1), loaded
+// together.  Its tests read the record word, then the distance, and only a
+// record that passes needs the sample; left to itself the compiler loads
+// each field inside the branch that uses it, three round trips to memory
+// in a row.  The empty asm consumes all eight words where they are loaded,
+// so both loads are issued back to back and one round trip serves the
+// tests (spatial reuse: 0.565 -> 0.555 ms at 1080p).
+__device__ __forceinline__ void load_record_sample(const ResPlanes& r, size_t i, uint4& r0, uint4& r1)
+{
+    uint4 s = r.r1[i];
+    uint4 c = r.r0[i];
+    asm volatile("" : "+v"(c.x), "+v"(c.y), "+v"(c.z), "+v"(c.w), "+v"(s.x), "+v"(s.y), "+v"(s.z), "+v"(s.w));
+    r0 = c, r1 = s;
+}
+
 // the record of reservoir o at the pixel with hit word hitw and distance
 // dist; shadow: kTraced / kVisible
 __device__ __forceinline__ void store_res(const ResPlanes& r, size_t i, const Res& o, uint32_t hitw, float dist,
@@ -555,13 +570,14 @@ __device__ __forceinline__ Merge spatial_merge(const FrameArgs& a, int x, int y)
         if ((nx == x && ny == y) || nx < 0 || ny < 0 || nx >= (int)a.W || ny >= (int)a.H) continue;
         const size_t ni = (size_t)ny * a.W + nx;
         // the neighbour's record word, distance, W and invPdf (16 bytes)
-        // hold all its tests; its sample is read only once they pass
-        const uint4 n0 = a.B.r0[ni];
+        // hold all its tests; its sample is needed only once they pass
+        uint4 n0, n1;
+        load_record_sample(a.B, ni, n0, n1);
         if ((n0.x & 0xffffffu) == 0u) continue;
         const float nd = __uint_as_float(n0.y);
         if (!same_face(rec_hit_bits(n0.x), g.hitw, a.axisOnlyFace) || !(fabsf(dist - nd) <= 0.1f * dist))
             continue;
-        Res nr = res_from(n0, a.B.r1[ni], make_float2(0.0f, 0.0f));  // wsum and pHat: the reweigh recomputes them
+        Res nr = res_from(n0, n1, make_float2(0.0f, 0.0f));  // wsum and pHat: the reweigh recomputes them
         if (nr.light >= a.L.n) continue;
         const float4 la = a.L.a[nr.light], lb = a.L.b[nr.light];
         const float ph = phat(a.point, nr.pos, face_normal(nr.face), s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w) *
