@@ -92,8 +92,9 @@ __device__ __forceinline__ void count_rays(unsigned long long* rays, int k, uint
 // different lights, so they take the wave-scheduled DDA: kPick picks the
 // walk per wave (trace_sched: uncounted unless a ray is out of bounds), else
 // the walk that counts its brick steps runs always (one copy of the loop:
-// in the spatial reuse + shading kernel a second copy costs more registers
-// than the count costs its fewer rays).  kNested: the nested DDA instead
+// for the shading pass's rays a second copy costs more registers than the
+// count costs their fewer rays; the MIS merge's kernel, with more registers
+// per lane, picks).  kNested: the nested DDA instead
 // (a stage's choice, kDdaNested: scenes whose rays stay coherent).
 template <bool kPick, bool kNested>
 __device__ __forceinline__ bool light_occluded(const WorldView& w, uint32_t lpos, uint32_t lsize, f3 spo, f3 L,
@@ -706,7 +707,7 @@ __device__ __forceinline__ float mis_weight_tile(const FrameArgs& a, int x, int 
             const f3 spo = f3{sh.sp.x + 0.1f * sh.N.x, sh.sp.y + 0.1f * sh.N.y, sh.sp.z + 0.1f * sh.N.z};
             const f3 Ld = normalize(sub(f3{smp.x, smp.y, smp.z}, spo));
             const float4 tla = a.L.a[light], tlb = a.L.b[light];
-            if (light_occluded<false, kNested>(a.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, a.shadowSteps))
+            if (light_occluded<true, kNested>(a.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, a.shadowSteps))
                 atomicOr(&L.occluded[owner], 1u << (tk >> 8));
         }
         __syncthreads();
