@@ -954,15 +954,16 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
     if (kMis) {
         Merge g{};
         if (in) g = spatial_merge<true>(a, x, y);
-        const float W = mis_weight_tile<kNested>(a, x, y, g, in);
+        // The merged reservoir is stored before its visibility rays are
+        // traced and its W written after them, so the fields the weight does
+        // not need are not held in registers across the traversal.
+        const size_t idx = (size_t)y * a.W + x;
         if (in) {
-            const size_t idx = (size_t)y * a.W + x;
-            if (g.merged) {
-                g.res.W = W;
-                a.cur.seed[idx] = g.seed;
-            }
+            if (g.merged) a.cur.seed[idx] = g.seed;
             store_res(a.A, idx, g.res, g.hitw, g.dist, g.shadow);
         }
+        const float W = mis_weight_tile<kNested>(a, x, y, g, in);
+        if (in && g.merged) a.A.r0[idx].z = __float_as_uint(W);
     } else if (in) {
         spatial_px(a, x, y);
     }
