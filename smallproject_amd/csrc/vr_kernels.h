@@ -128,7 +128,25 @@ constexpr uint32_t kDdaScheduled = 0u, kDdaNested = 1u;
 // live values, and runs best at 5.
 constexpr uint32_t kDefaultWaves[3] = {5, 7, 8};
 constexpr uint32_t kDefaultWavesMis = 5;
+// the split MIS merge's merge/list and weights/shading kernels (the tracer
+// runs at 8)
+constexpr uint32_t kDefaultWavesMisSplit = 8;
 hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s);
+
+// The MIS merge of one view in three launches: the merge, which stores the
+// merged reservoirs and lists the visibility rays its weights need; a
+// persistent tracer over that list; the weights and the shading.  The list
+// holds up to W*H*spatialTaps rays (owner pixel << 5 | tap, neighbour
+// pixel).  `count` is zero before the first launch; the last launch sets it
+// to zero again for the next frame.
+struct MisBufs {
+    uint2* list;
+    uint32_t* count;  // [1] rays listed
+    uint32_t* occ;    // [W*H] bit k: tap k's ray found the merged sample occluded
+    uint4* info;      // [W*H] stream before the taps, combined taps, sel + 1, own M
+};
+// traceBlocks: workgroups of the tracer's persistent grid
+hipError_t launch_mis_split(const FrameArgs& a, const MisBufs& m, uint32_t traceBlocks, hipStream_t s);
 // Up to kChunkViews views rendered by one launch per stage: their
 // FrameArgs travel in the kernel argument (8 x 448 B + the tile table,
 // under the 4 KiB kernel-argument limit), so pointers in them are global

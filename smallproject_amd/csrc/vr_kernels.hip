@@ -945,28 +945,15 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_initial(FrameArgs a)
 // thread reaches every barrier (the MIS weight's included).
 // kPark: shading parks its live values in LDS across the traversal (not
 // where a kernel also holds the MIS merge's LDS)
-template <bool kMis, bool kNested, bool kPark = !kMis>
-__device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
+// Shading of a 16x16 tile: the pixels whose final sample needs a shadow ray
+// that the candidate pass did not trace are gathered in LDS and shaded by
+// the first threads, so their rays fill as few waves as possible.  Every
+// thread of the workgroup calls it (barriers).
+template <bool kPark, bool kNested>
+__device__ __forceinline__ void shade_tile(const FrameArgs& a, int x, int y, bool in)
 {
     __shared__ uint32_t rayPx[256];  // threads of the tile whose pixel needs a ray
     __shared__ uint32_t waveRays[4];
-    const bool in = x < (int)a.W && y < (int)a.H;
-    if (kMis) {
-        Merge g{};
-        if (in) g = spatial_merge<true>(a, x, y);
-        // The merged reservoir is stored before its visibility rays are
-        // traced and its W written after them, so the fields the weight does
-        // not need are not held in registers across the traversal.
-        const size_t idx = (size_t)y * a.W + x;
-        if (in) {
-            if (g.merged) a.cur.seed[idx] = g.seed;
-            store_res(a.A, idx, g.res, g.hitw, g.dist, g.shadow);
-        }
-        const float W = mis_weight_tile<kNested>(a, x, y, g, in);
-        if (in && g.merged) a.A.r0[idx].z = __float_as_uint(W);
-    } else if (in) {
-        spatial_px(a, x, y);
-    }
     const bool defer = in && shade_px<true, kPark, kNested>(a, x, y);
     const uint64_t m = __ballot(defer);
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
@@ -988,6 +975,29 @@ __device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
     }
 }
 
+template <bool kMis, bool kNested, bool kPark = !kMis>
+__device__ __forceinline__ void resolve_tile(const FrameArgs& a, int x, int y)
+{
+    const bool in = x < (int)a.W && y < (int)a.H;
+    if (kMis) {
+        Merge g{};
+        if (in) g = spatial_merge<true>(a, x, y);
+        // The merged reservoir is stored before its visibility rays are
+        // traced and its W written after them, so the fields the weight does
+        // not need are not held in registers across the traversal.
+        const size_t idx = (size_t)y * a.W + x;
+        if (in) {
+            if (g.merged) a.cur.seed[idx] = g.seed;
+            store_res(a.A, idx, g.res, g.hitw, g.dist, g.shadow);
+        }
+        const float W = mis_weight_tile<kNested>(a, x, y, g, in);
+        if (in && g.merged) a.A.r0[idx].z = __float_as_uint(W);
+    } else if (in) {
+        spatial_px(a, x, y);
+    }
+    shade_tile<kPark, kNested>(a, x, y, in);
+}
+
 template <bool kMis, int W, bool kNested>
 __global__ __launch_bounds__(256) VR_WAVES(W) void k_resolve(FrameArgs a)
 {
@@ -997,6 +1007,189 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_resolve(FrameArgs a)
     FrameArgs b = a;
     b.w.uber = uber_lds();
     resolve_tile<kMis, kNested>(b, x, y);
+}
+
+// ---------------------------------------------------------------------------
+// The MIS merge in three launches (one view).  The weights' visibility rays
+// are listed in device memory and traced by a persistent grid of compact
+// waves, instead of inside the merge kernel, whose registers and LDS allow
+// 5 waves per SIMD.  Every operation is mis_weight_tile's, in its order.
+// ---------------------------------------------------------------------------
+
+// Combined tap i of a pixel's MIS weight for its merged sample (res, with
+// the sample's face normal Nl and light records la, lb): the neighbour's
+// target T (0 when the candidate pass found the sample occluded from there),
+// whether a ray must decide its visibility, its pixel and its M.  `rs` is
+// the replayed stream at tap i's offset draws.
+struct MisTap {
+    float p;
+    bool ray;
+    uint32_t ni, M;
+};
+
+__device__ __forceinline__ MisTap mis_tap(const FrameArgs& a, int x, int y, const Res& res, f3 Nl, float4 la,
+                                          float4 lb, uint32_t& rs, float rad)
+{
+    int dx, dy;
+    tap_offset(rs, rad, dx, dy);
+    (void)rnd(rs);  // the merge's draw
+    const int nx = x + dx, ny = y + dy;
+    const size_t ni = (size_t)ny * a.W + nx;
+    const uint32_t nh = a.cur.hitw[ni];
+    const uint4 n0 = a.B.r0[ni];
+    const Shading t = shading_at(a, nx, ny, nh, __uint_as_float(n0.y));
+    MisTap o;
+    o.p = phat(a.point, res.pos, Nl, t.sp, t.N, t.brdf, f3{lb.x, lb.y, lb.z}, la.w);
+    o.ray = false;
+    if (o.p > 0.0f) {
+        const int known = known_visibility(n0.x, a.B.r1[ni], res.light, res.pos);
+        if (known == 0) o.p = 0.0f;
+        o.ray = known < 0;
+    }
+    o.ni = (uint32_t)ni;
+    o.M = n0.x & 0xffffffu;
+    return o;
+}
+
+// skip tap i's offset draws (a tap that was not combined)
+__device__ __forceinline__ void mis_skip_tap(uint32_t& rs, float rad)
+{
+    int dx, dy;
+    tap_offset(rs, rad, dx, dy);
+}
+
+template <int W>
+__global__ __launch_bounds__(256) VR_WAVES(W) void k_mis_list(FrameArgs a, MisBufs m)
+{
+    int x, y;
+    tile_pixel(a.tileOrder, x, y);
+    const bool in = x < (int)a.W && y < (int)a.H;
+    const size_t idx = in ? (size_t)y * a.W + x : 0u;
+    uint32_t rays = 0u;  // taps whose visibility ray is listed
+    Merge g{};
+    const float rad = (float)(a.spatialRadius - 1u);
+    float4 la = make_float4(0.0f, 0.0f, 0.0f, 0.0f), lb = la;
+    f3 Nl = f3{0.0f, 0.0f, 0.0f};
+    bool need = false;
+    if (in) {
+        g = spatial_merge<true>(a, x, y);
+        if (g.merged) a.cur.seed[idx] = g.seed;
+        store_res(a.A, idx, g.res, g.hitw, g.dist, g.shadow);  // W: k_mis_finish
+        m.info[idx] = make_uint4(g.seed0, g.mask, (uint32_t)(g.sel + 1), g.M0);
+        m.occ[idx] = 0u;
+        need = g.merged && g.res.wsum > 0.0f && g.res.light < a.L.n;
+        if (need) {
+            la = a.L.a[g.res.light], lb = a.L.b[g.res.light];
+            Nl = face_normal(g.res.face);
+            uint32_t rs = g.seed0;
+            for (uint32_t i = 0; i < a.spatialTaps && (g.mask >> i); i++) {
+                if (!((g.mask >> i) & 1u)) {
+                    mis_skip_tap(rs, rad);
+                    continue;
+                }
+                if (mis_tap(a, x, y, g.res, Nl, la, lb, rs, rad).ray) rays |= 1u << i;
+            }
+        }
+    }
+    // one atomic per wave: the lanes' rays go to consecutive slots
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t n = (uint32_t)__popc(rays);
+    uint32_t incl = n;
+    for (uint32_t o = 1u; o < 64u; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0u;
+    if (lane == 63u && total != 0u) base = atomicAdd(m.count, total);
+    base = __shfl(base, 63, 64);
+    if (rays != 0u) {
+        // replay the offsets of the listed taps
+        uint32_t slot = base + incl - n;
+        uint32_t rs = g.seed0;
+        for (uint32_t i = 0; i < a.spatialTaps && (rays >> i); i++) {
+            int dx, dy;
+            tap_offset(rs, rad, dx, dy);
+            if (!((g.mask >> i) & 1u)) continue;
+            (void)rnd(rs);
+            if ((rays >> i) & 1u)
+                m.list[slot++] = make_uint2(((uint32_t)idx << 5) | i, (uint32_t)((size_t)(y + dy) * a.W + (x + dx)));
+        }
+    }
+}
+
+template <bool kNested>
+__global__ __launch_bounds__(256) VR_WAVES(8) void k_mis_trace(FrameArgs a, MisBufs m)
+{
+    stage_uber(a.w.uber);
+    FrameArgs b = a;
+    b.w.uber = uber_lds();
+    const uint32_t n = *m.count;
+    if (blockIdx.x == 0u && threadIdx.x == 0u) count_rays(b.rays, kRaysMis, n);
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < n; t += gridDim.x * 256u) {
+        const uint2 task = m.list[t];
+        const uint32_t owner = task.x >> 5, k = task.x & 31u, ni = task.y;
+        const uint4 smp = b.A.r1[owner];  // the merged sample, stored by k_mis_list
+        const int nx = (int)(ni % b.W), ny = (int)(ni / b.W);
+        const float nd = __uint_as_float(b.B.r0[ni].y);
+        const Shading sh = shading_at(b, nx, ny, b.cur.hitw[ni], nd);
+        const f3 spo = f3{sh.sp.x + 0.1f * sh.N.x, sh.sp.y + 0.1f * sh.N.y, sh.sp.z + 0.1f * sh.N.z};
+        const f3 Ld = normalize(sub(f3{__uint_as_float(smp.x), __uint_as_float(smp.y), __uint_as_float(smp.z)}, spo));
+        const float4 tla = b.L.a[smp.w], tlb = b.L.b[smp.w];
+        if (light_occluded<true, kNested>(b.w, __float_as_uint(tlb.w), (uint32_t)tla.w, spo, Ld, b.shadowSteps))
+            atomicOr(&m.occ[owner], 1u << k);
+    }
+}
+
+template <int W, bool kNested>
+__global__ __launch_bounds__(256) VR_WAVES(W) void k_mis_finish(FrameArgs a, MisBufs m)
+{
+    stage_uber(a.w.uber);
+    int x, y;
+    tile_pixel(a.tileOrder, x, y);
+    FrameArgs b = a;
+    b.w.uber = uber_lds();
+    if (blockIdx.x == 0u && blockIdx.y == 0u && threadIdx.x == 0u) *m.count = 0u;  // the tracer has read it
+    const bool in = x < (int)b.W && y < (int)b.H;
+    if (in) {
+        const size_t idx = (size_t)y * b.W + x;
+        const uint32_t hitw = b.cur.hitw[idx];
+        if ((hitw & 0xffffffu) != 0u && b.spatial) {
+            // mis_weight_tile's weight of the merged reservoir
+            const Res res = load_res(b.A, idx);
+            const uint4 info = m.info[idx];
+            const uint32_t mask = info.y, occ = m.occ[idx];
+            const int sel = (int)info.z - 1;
+            float Wm = 0.0f;
+            if (res.wsum > 0.0f && res.light < b.L.n) {
+                const float dist = __uint_as_float(b.A.r0[idx].y);
+                const Shading s = shading_at(b, x, y, hitw, dist);
+                const float4 la = b.L.a[res.light], lb = b.L.b[res.light];
+                const f3 Nl = face_normal(res.face);
+                const float p0 = phat(b.point, res.pos, Nl, s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w);
+                float den = (float)info.w * p0;
+                float psel = sel < 0 ? p0 : 0.0f;
+                const float rad = (float)(b.spatialRadius - 1u);
+                uint32_t rs = info.x;
+                for (uint32_t i = 0; i < b.spatialTaps && (mask >> i); i++) {
+                    if (!((mask >> i) & 1u)) {
+                        mis_skip_tap(rs, rad);
+                        continue;
+                    }
+                    const MisTap t = mis_tap(b, x, y, res, Nl, la, lb, rs, rad);
+                    const float term = (float)t.M * t.p;
+                    const bool blocked = t.ray && ((occ >> i) & 1u);
+                    den += blocked ? 0.0f : term;
+                    if ((int)i == sel) psel = blocked ? 0.0f : t.p;
+                }
+                const float d = res.pHat * den;
+                const float Wq = (res.wsum * psel) / d;
+                Wm = (d > 0.0f && isfinite(Wq)) ? Wq : 0.0f;
+            }
+            b.A.r0[idx].z = __float_as_uint(Wm);
+        }
+    }
+    shade_tile<false, kNested>(b, x, y, in);
 }
 
 template <bool kPersistent>
@@ -1179,6 +1372,42 @@ hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s)
     if (!k) return hipErrorInvalidValue;
     void* args[] = {const_cast<FrameArgs*>(&a)};
     return hipLaunchKernel(k, frame_grid(a.W, a.H), dim3(256), args, 0, s);
+}
+
+template <int W>
+static const void* mis_kernel_w(int which, bool nested)
+{
+    if (which == 0) return reinterpret_cast<const void*>(&k_mis_list<W>);
+    return nested ? reinterpret_cast<const void*>(&k_mis_finish<W, true>)
+                  : reinterpret_cast<const void*>(&k_mis_finish<W, false>);
+}
+
+// which: 0 the merge and ray list, 1 the weights and shading
+static const void* mis_kernel(int which, uint32_t waves, bool nested)
+{
+    switch (waves == 0 ? kDefaultWavesMisSplit : waves) {
+    case 5: return mis_kernel_w<5>(which, nested);
+    case 6: return mis_kernel_w<6>(which, nested);
+    case 7: return mis_kernel_w<7>(which, nested);
+    case 8: return mis_kernel_w<8>(which, nested);
+    default: return nullptr;
+    }
+}
+
+hipError_t launch_mis_split(const FrameArgs& a, const MisBufs& m, uint32_t traceBlocks, hipStream_t s)
+{
+    if (a.dda > kDdaNested || traceBlocks == 0u) return hipErrorInvalidValue;
+    const bool nested = a.dda == kDdaNested;
+    const void* k0 = mis_kernel(0, a.waves, nested);
+    const void* k2 = mis_kernel(1, a.waves, nested);
+    if (!k0 || !k2) return hipErrorInvalidValue;
+    const void* k1 = nested ? reinterpret_cast<const void*>(&k_mis_trace<true>)
+                            : reinterpret_cast<const void*>(&k_mis_trace<false>);
+    void* args[] = {const_cast<FrameArgs*>(&a), const_cast<MisBufs*>(&m)};
+    hipError_t e = hipLaunchKernel(k0, frame_grid(a.W, a.H), dim3(256), args, 0, s);
+    if (e == hipSuccess) e = hipLaunchKernel(k1, dim3(traceBlocks), dim3(256), args, 0, s);
+    if (e == hipSuccess) e = hipLaunchKernel(k2, frame_grid(a.W, a.H), dim3(256), args, 0, s);
+    return e;
 }
 
 template <bool kP>

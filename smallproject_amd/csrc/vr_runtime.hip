@@ -153,6 +153,14 @@ struct vr_ctx {
     Camera lastCam{};             // camera of the last rendered frame
     uint32_t stageWaves[VR_NUM_STAGES] = {};  // occupancy target per stage (0: default)
     uint32_t stageDda[VR_NUM_STAGES] = {};    // VR_DDA_* of each stage's shadow rays
+    // the MIS merge's visibility-ray list and its per-pixel state (allocated
+    // by the first MIS frame, the list regrown for more taps)
+    uint2* misList = nullptr;
+    size_t misCap = 0;             // rays the list holds
+    uint32_t* misCount = nullptr;  // [1]
+    uint32_t* misOcc = nullptr;    // [W*H]
+    uint4* misInfo = nullptr;      // [W*H]
+    uint32_t traceBlocks = 0;      // the tracer's persistent grid
     // scene motion on the device (allocated by vr_set_light_motion)
     float4* lightPos = nullptr;
     float4* lightVel = nullptr;
@@ -558,6 +566,7 @@ int vr_destroy(vr_ctx* c)
     dfree(c->clusterStart), dfree(c->clusterOf), dfree(c->clusterPick), dfree(c->clusterPickShare);
     dfree(c->clusterPower);
     dfree(c->candSlot);
+    dfree(c->misList), dfree(c->misCount), dfree(c->misOcc), dfree(c->misInfo);
     dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
     dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget), dfree(c->claim);
     for (int i = 0; i < 2 * VR_NUM_STAGES; i++)
@@ -678,6 +687,43 @@ static int check_frame(vr_ctx* c, const vr_render_params* p, const char* who)
     return VR_OK;
 }
 
+// The MIS merge's ray list for `taps` taps per pixel.  Pixel indices share
+// a list word with the tap index (5 bits), so frames of more than 2^27
+// pixels keep the one-kernel merge (mis_split false).  Regrowing waits for
+// the frames in flight, which may still read the old list.
+static int mis_alloc(vr_ctx* c, uint32_t taps, bool& mis_split)
+{
+    const size_t npx = (size_t)c->cfg.width * c->cfg.height;
+    mis_split = npx <= ((size_t)1 << 27);
+    if (!mis_split || taps == 0u) return VR_OK;
+    const size_t need = npx * taps;
+    if (c->misList && c->misCap >= need) return VR_OK;
+    if (!c->misCount) {
+        int cus = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        c->traceBlocks = (uint32_t)(cus > 0 ? cus : 1) * 8u;  // 8 workgroups of 4 waves per CU
+        if (dalloc(&c->misCount, 1) != hipSuccess || dalloc(&c->misOcc, npx) != hipSuccess ||
+            dalloc(&c->misInfo, npx) != hipSuccess) {
+            dfree(c->misCount), dfree(c->misOcc), dfree(c->misInfo);
+            c->misCount = nullptr, c->misOcc = nullptr, c->misInfo = nullptr;
+            return fail(c, VR_ERR_NOMEM, "MIS merge: device allocation failed");
+        }
+        HIPCHK(c, hipMemset(c->misCount, 0, sizeof(uint32_t)));
+    }
+    if (c->misList) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->side) HIPCHK(c, hipStreamSynchronize(c->side));
+        dfree(c->misList);
+        c->misList = nullptr, c->misCap = 0;
+    }
+    if (dalloc(&c->misList, need) != hipSuccess) {
+        c->misList = nullptr;
+        return fail(c, VR_ERR_NOMEM, "MIS merge: device allocation of the ray list failed");
+    }
+    c->misCap = need;
+    return VR_OK;
+}
+
 int vr_render_frame(vr_ctx* c, const vr_render_params* p)
 {
     if (!c || !p) return VR_ERR_ARG;
@@ -686,6 +732,12 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         if (rc != VR_OK) return rc;
     }
     const bool learned = c->cfg.source_pdf == VR_SOURCE_LEARNED;
+    bool misSplit = false;
+    if ((c->cfg.flags & VR_FLAG_MIS_SPATIAL) && p->spatial) {
+        const int rc = mis_alloc(c, p->spatialTaps, misSplit);
+        if (rc != VR_OK) return rc;
+        misSplit = misSplit && c->misList != nullptr;
+    }
     const int slot = (int)(c->frameNo % kRing);
     {
         // before any state changes: a failure here leaves the context as it was
@@ -726,7 +778,12 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         if (!(propose && overlap)) HIPCHK(c, hipEventRecord(c->ev[slot][2 * s], st));
         if (propose && !overlap)
             HIPCHK(c, launch_proposal(as, mlp_view(c), c->candSlot, proposal_salt(p->R0), c->stream));
-        HIPCHK(c, launch_stage(s, as, st));
+        if (s == VR_STAGE_RESOLVE && misSplit) {
+            const MisBufs mb{c->misList, c->misCount, c->misOcc, c->misInfo};
+            HIPCHK(c, launch_mis_split(as, mb, c->traceBlocks, st));
+        } else {
+            HIPCHK(c, launch_stage(s, as, st));
+        }
         HIPCHK(c, hipEventRecord(c->ev[slot][2 * s + 1], st));
         if (s == VR_STAGE_INITIAL) {
             HIPCHK(c, hipEventRecord(c->initDone, c->stream));
