@@ -135,7 +135,7 @@ hipError_t launch_stage(int stage, const FrameArgs& a, hipStream_t s);
 
 // The MIS merge of one view in three launches: the merge, which stores the
 // merged reservoirs and lists the visibility rays its weights need; a
-// persistent tracer over that list; the weights and the shading.  The list
+// tracer over that list, one ray per thread; the weights and the shading.  The list
 // holds up to W*H*spatialTaps rays (owner pixel << 5 | tap, neighbour
 // pixel).  `count` is zero before the first launch; the last launch sets it
 // to zero again for the next frame.
@@ -145,7 +145,8 @@ struct MisBufs {
     uint32_t* occ;    // [W*H] bit k: tap k's ray found the merged sample occluded
     uint4* info;      // [W*H] stream before the taps, combined taps, sel + 1, own M
 };
-// traceBlocks: workgroups of the tracer's persistent grid
+// traceBlocks: workgroups of the tracer's grid (256 rays each), enough for
+// the list's capacity
 hipError_t launch_mis_split(const FrameArgs& a, const MisBufs& m, uint32_t traceBlocks, hipStream_t s);
 // Up to kChunkViews views rendered by one launch per stage: their
 // FrameArgs travel in the kernel argument (8 x 448 B + the tile table,

@@ -1011,9 +1011,9 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_resolve(FrameArgs a)
 
 // ---------------------------------------------------------------------------
 // The MIS merge in three launches (one view).  The weights' visibility rays
-// are listed in device memory and traced by a persistent grid of compact
-// waves, instead of inside the merge kernel, whose registers and LDS allow
-// 5 waves per SIMD.  Every operation is mis_weight_tile's, in its order.
+// are listed in device memory and traced one per thread by compact waves,
+// instead of inside the merge kernel, whose registers and LDS allow 5 waves
+// per SIMD.  Every operation is mis_weight_tile's, in its order.
 // ---------------------------------------------------------------------------
 
 // Combined tap i of a pixel's MIS weight for its merged sample (res, with
@@ -1121,10 +1121,13 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_mis_list(FrameArgs a, MisBu
 template <bool kNested>
 __global__ __launch_bounds__(256) VR_WAVES(8) void k_mis_trace(FrameArgs a, MisBufs m)
 {
+    // one thread per listed ray: the grid covers the list's capacity and the
+    // workgroups past the count leave at once (workgroup 0 counts the rays)
+    const uint32_t n = *m.count;
+    if (blockIdx.x * 256u >= n && blockIdx.x != 0u) return;
     stage_uber(a.w.uber);
     FrameArgs b = a;
     b.w.uber = uber_lds();
-    const uint32_t n = *m.count;
     if (blockIdx.x == 0u && threadIdx.x == 0u) count_rays(b.rays, kRaysMis, n);
     for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < n; t += gridDim.x * 256u) {
         const uint2 task = m.list[t];

@@ -160,7 +160,6 @@ struct vr_ctx {
     uint32_t* misCount = nullptr;  // [1]
     uint32_t* misOcc = nullptr;    // [W*H]
     uint4* misInfo = nullptr;      // [W*H]
-    uint32_t traceBlocks = 0;      // the tracer's persistent grid
     // scene motion on the device (allocated by vr_set_light_motion)
     float4* lightPos = nullptr;
     float4* lightVel = nullptr;
@@ -699,9 +698,6 @@ static int mis_alloc(vr_ctx* c, uint32_t taps, bool& mis_split)
     const size_t need = npx * taps;
     if (c->misList && c->misCap >= need) return VR_OK;
     if (!c->misCount) {
-        int cus = 0;
-        HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-        c->traceBlocks = (uint32_t)(cus > 0 ? cus : 1) * 8u;  // 8 workgroups of 4 waves per CU
         if (dalloc(&c->misCount, 1) != hipSuccess || dalloc(&c->misOcc, npx) != hipSuccess ||
             dalloc(&c->misInfo, npx) != hipSuccess) {
             dfree(c->misCount), dfree(c->misOcc), dfree(c->misInfo);
@@ -780,7 +776,7 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
             HIPCHK(c, launch_proposal(as, mlp_view(c), c->candSlot, proposal_salt(p->R0), c->stream));
         if (s == VR_STAGE_RESOLVE && misSplit) {
             const MisBufs mb{c->misList, c->misCount, c->misOcc, c->misInfo};
-            HIPCHK(c, launch_mis_split(as, mb, c->traceBlocks, st));
+            HIPCHK(c, launch_mis_split(as, mb, (uint32_t)((c->misCap + 255) / 256), st));
         } else {
             HIPCHK(c, launch_stage(s, as, st));
         }
