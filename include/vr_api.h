@@ -107,8 +107,8 @@ typedef struct vr_render_params {
  * target of s's pixel times its visibility of y (one shadow ray per
  * combined neighbour; the own term has no ray).  At most VR_MIS_MAX_TAPS
  * spatial taps.  The first frame with the merge allocates the visibility
- * rays' list (8 bytes per pixel and tap, plus 20 bytes per pixel), as does
- * a later frame with more taps. */
+ * rays' list and the taps' terms (16 bytes per pixel and tap, plus 20 bytes
+ * per pixel), as does a later frame with more taps. */
 #define VR_FLAG_MIS_SPATIAL (1u << 8)
 #define VR_MIS_MAX_TAPS 32
 

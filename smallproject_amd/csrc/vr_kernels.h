@@ -143,7 +143,8 @@ struct MisBufs {
     uint2* list;
     uint32_t* count;  // [1] rays listed
     uint32_t* occ;    // [W*H] bit k: tap k's ray found the merged sample occluded
-    uint4* info;      // [W*H] stream before the taps, combined taps, sel + 1, own M
+    uint4* info;      // [W*H] taps whose ray is listed, combined taps, sel + 1, own M
+    float2* terms;    // [tap][W*H] M * T and T of each combined tap
 };
 // traceBlocks: workgroups of the tracer's grid (256 rays each), enough for
 // the list's capacity

@@ -1075,21 +1075,25 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_mis_list(FrameArgs a, MisBu
         g = spatial_merge<true>(a, x, y);
         if (g.merged) a.cur.seed[idx] = g.seed;
         store_res(a.A, idx, g.res, g.hitw, g.dist, g.shadow);  // W: k_mis_finish
-        m.info[idx] = make_uint4(g.seed0, g.mask, (uint32_t)(g.sel + 1), g.M0);
         m.occ[idx] = 0u;
         need = g.merged && g.res.wsum > 0.0f && g.res.light < a.L.n;
         if (need) {
             la = a.L.a[g.res.light], lb = a.L.b[g.res.light];
             Nl = face_normal(g.res.face);
+            const size_t npx = (size_t)a.W * a.H;
             uint32_t rs = g.seed0;
             for (uint32_t i = 0; i < a.spatialTaps && (g.mask >> i); i++) {
                 if (!((g.mask >> i) & 1u)) {
                     mis_skip_tap(rs, rad);
                     continue;
                 }
-                if (mis_tap(a, x, y, g.res, Nl, la, lb, rs, rad).ray) rays |= 1u << i;
+                const MisTap t = mis_tap(a, x, y, g.res, Nl, la, lb, rs, rad);
+                // the term as the one-kernel merge adds it, (float)M * T
+                m.terms[i * npx + idx] = make_float2((float)t.M * t.p, t.p);
+                if (t.ray) rays |= 1u << i;
             }
         }
+        m.info[idx] = make_uint4(rays, g.mask, (uint32_t)(g.sel + 1), g.M0);
     }
     // one atomic per wave: the lanes' rays go to consecutive slots
     const uint32_t lane = threadIdx.x & 63u;
@@ -1161,7 +1165,7 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_mis_finish(FrameArgs a, Mis
             // mis_weight_tile's weight of the merged reservoir
             const Res res = load_res(b.A, idx);
             const uint4 info = m.info[idx];
-            const uint32_t mask = info.y, occ = m.occ[idx];
+            const uint32_t rays = info.x, mask = info.y, occ = m.occ[idx];
             const int sel = (int)info.z - 1;
             float Wm = 0.0f;
             if (res.wsum > 0.0f && res.light < b.L.n) {
@@ -1172,18 +1176,14 @@ __global__ __launch_bounds__(256) VR_WAVES(W) void k_mis_finish(FrameArgs a, Mis
                 const float p0 = phat(b.point, res.pos, Nl, s.sp, s.N, s.brdf, f3{lb.x, lb.y, lb.z}, la.w);
                 float den = (float)info.w * p0;
                 float psel = sel < 0 ? p0 : 0.0f;
-                const float rad = (float)(b.spatialRadius - 1u);
-                uint32_t rs = info.x;
+                // the combined taps' terms, stored by k_mis_list, in tap order
+                const size_t npx = (size_t)b.W * b.H;
                 for (uint32_t i = 0; i < b.spatialTaps && (mask >> i); i++) {
-                    if (!((mask >> i) & 1u)) {
-                        mis_skip_tap(rs, rad);
-                        continue;
-                    }
-                    const MisTap t = mis_tap(b, x, y, res, Nl, la, lb, rs, rad);
-                    const float term = (float)t.M * t.p;
-                    const bool blocked = t.ray && ((occ >> i) & 1u);
-                    den += blocked ? 0.0f : term;
-                    if ((int)i == sel) psel = blocked ? 0.0f : t.p;
+                    if (!((mask >> i) & 1u)) continue;
+                    const float2 t = m.terms[i * npx + idx];
+                    const bool blocked = ((rays >> i) & 1u) && ((occ >> i) & 1u);
+                    den += blocked ? 0.0f : t.x;
+                    if ((int)i == sel) psel = blocked ? 0.0f : t.y;
                 }
                 const float d = res.pHat * den;
                 const float Wq = (res.wsum * psel) / d;

@@ -160,6 +160,7 @@ struct vr_ctx {
     uint32_t* misCount = nullptr;  // [1]
     uint32_t* misOcc = nullptr;    // [W*H]
     uint4* misInfo = nullptr;      // [W*H]
+    float2* misTerms = nullptr;    // [taps][W*H], with the list
     // scene motion on the device (allocated by vr_set_light_motion)
     float4* lightPos = nullptr;
     float4* lightVel = nullptr;
@@ -565,7 +566,7 @@ int vr_destroy(vr_ctx* c)
     dfree(c->clusterStart), dfree(c->clusterOf), dfree(c->clusterPick), dfree(c->clusterPickShare);
     dfree(c->clusterPower);
     dfree(c->candSlot);
-    dfree(c->misList), dfree(c->misCount), dfree(c->misOcc), dfree(c->misInfo);
+    dfree(c->misList), dfree(c->misCount), dfree(c->misOcc), dfree(c->misInfo), dfree(c->misTerms);
     dfree(c->trainX), dfree(c->trainT), dfree(c->trainW), dfree(c->partial), dfree(c->chunkLoss);
     dfree(c->lightPos), dfree(c->lightVel), dfree(c->lightVoxel), dfree(c->lightTarget), dfree(c->claim);
     for (int i = 0; i < 2 * VR_NUM_STAGES; i++)
@@ -709,11 +710,12 @@ static int mis_alloc(vr_ctx* c, uint32_t taps, bool& mis_split)
     if (c->misList) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->side) HIPCHK(c, hipStreamSynchronize(c->side));
-        dfree(c->misList);
-        c->misList = nullptr, c->misCap = 0;
+        dfree(c->misList), dfree(c->misTerms);
+        c->misList = nullptr, c->misTerms = nullptr, c->misCap = 0;
     }
-    if (dalloc(&c->misList, need) != hipSuccess) {
-        c->misList = nullptr;
+    if (dalloc(&c->misList, need) != hipSuccess || dalloc(&c->misTerms, need) != hipSuccess) {
+        dfree(c->misList), dfree(c->misTerms);
+        c->misList = nullptr, c->misTerms = nullptr;
         return fail(c, VR_ERR_NOMEM, "MIS merge: device allocation of the ray list failed");
     }
     c->misCap = need;
@@ -775,7 +777,7 @@ int vr_render_frame(vr_ctx* c, const vr_render_params* p)
         if (propose && !overlap)
             HIPCHK(c, launch_proposal(as, mlp_view(c), c->candSlot, proposal_salt(p->R0), c->stream));
         if (s == VR_STAGE_RESOLVE && misSplit) {
-            const MisBufs mb{c->misList, c->misCount, c->misOcc, c->misInfo};
+            const MisBufs mb{c->misList, c->misCount, c->misOcc, c->misInfo, c->misTerms};
             HIPCHK(c, launch_mis_split(as, mb, (uint32_t)((c->misCap + 255) / 256), st));
         } else {
             HIPCHK(c, launch_stage(s, as, st));
