@@ -121,7 +121,7 @@ class OracleMirror:
 
 def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, height: int | None = None,
                quirks: int | None = None, source: int | None = None, check_every: bool = True, setup=None,
-               flags: int | None = None, before_frame=None, strict: bool = True):
+               flags: int | None = None, before_frame=None, strict: bool = True, params=None):
     """Render `frames` frames of a configuration on the GPU and in the oracle
     with identical per-frame params; returns a Comparison per checked frame.
 
@@ -130,7 +130,8 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
     stage downstream of the proposal is compared exactly.  `setup(world,
     renderer)` runs once after the world is on the device;
     `before_frame(f, world, renderer, view, frames)` before each frame (the
-    oracle's world view and frame state).  `strict` (the default) compares
+    oracle's world view and frame state); `params(f, p)` may change frame
+    f's render params, which both sides then take.  `strict` (the default) compares
     the float fields bit for bit too; strict=False accepts them within the
     FP32 tolerance above."""
     import dataclasses
@@ -152,6 +153,8 @@ def run_config(sc: configs.SceneConfig, frames: int, width: int | None = None, h
         w.look_at(pos, tgt)
         p = w.prepare_frame()
         w.finish_frame()
+        if params is not None:
+            params(f, p)
         r.render(p)
         mirror.render(p, r)
         if not (check_every or f == frames - 1):

@@ -171,6 +171,22 @@ def test_mis_spatial_merge_many_taps(gpu, taps, radius):
     assert all(c.ok for c in res), report
 
 
+def test_mis_ray_list_grows_with_the_taps(gpu):
+    """The MIS merge's visibility-ray list is sized by the first frame's taps
+    and regrown when a later frame takes more: 2, 8, 32, then 4 taps in one
+    context, every frame bit-exact against the oracle."""
+    taps = [2, 8, 32, 4]
+
+    def per_frame(f, p):
+        p.spatialTaps = taps[f]
+
+    res = run_config(configs.CONFIGS["c2_area_1024"], frames=len(taps), width=160, height=96,
+                     flags=_lib.FLAG_MIS_SPATIAL, params=per_frame)
+    report = "\n".join(c.report() for c in res)
+    print(report)
+    assert all(c.ok for c in res), report
+
+
 @pytest.mark.parametrize("flags", [0, _lib.FLAG_MIS_SPATIAL], ids=["biased", "mis"])
 def test_world_without_lights_matches_oracle(gpu, flags):
     """A world with no emitters: every candidate, reuse and shading guard on
